@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Plain SpGEMM throughput on one MI355X: GFLOP/s = 2 * P / t.
+
+A step is one full product C = A * A^T of the workload (symbolic + routing +
+numeric, C taken from the context pool), with A resident in HBM before the
+timed region starts.  P = number of intermediate products (the reference
+prints P / t, src/ams_hybaccspgemm.c:370-388; this line doubles it).
+
+Default workload: skewed R-MAT (a=0.57 b=0.19 c=0.19) at scale 19, edge
+factor 32, symmetric, loop- and duplicate-free, FP64 values 1.0, C = A * A^T.
+
+Also reported on the same line:
+  roofline      the dominant kernel's algorithmic HBM bytes per launch over
+                its event-timed average duration, against 8 TB/s
+  cpu_baseline  the CPU oracle (a C/OpenMP restatement of the reference's AMS
+                hybacc path) on a bounded row sample of the same matrix
+  checks        the reference's VERIFYRESULT triple of C and, for all-ones
+                inputs, sum(C) == P
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "contention-aware-spgemm_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "SpGEMM GFLOP/s (2·flops/t) plain vs batched & fwd+bwd, 1 MI355X, bit-exact"
+HBM_PEAK = 8.0e12
+
+WORKLOADS = {
+    "g500_s19_ef32": dict(kind="rmat", scale=19, ef=32, abc="G500",
+                          text="G500 R-MAT s19 ef32 (a=0.57 b=0.19 c=0.19), symmetric, loop/dup-free, "
+                               "FP64 1.0, C = A*A^T"),
+    "er_s20_ef16": dict(kind="rmat", scale=20, ef=16, abc="ER",
+                        text="uniform R-MAT s20 ef16 (a=b=c=d=0.25), symmetric, loop/dup-free, FP64 1.0, "
+                             "C = A*A^T"),
+    "g500_s16_ef16": dict(kind="rmat", scale=16, ef=16, abc="G500",
+                          text="G500 R-MAT s16 ef16, symmetric, loop/dup-free, FP64 1.0, C = A*A^T"),
+    "join_agg": dict(kind="join",
+                     text="join-aggregate incidence 850000 x 20000 (lognormal(2,1.5) degrees capped 5000, "
+                          "Zipf(1.2) columns), FP64 1.0, C = A^T*A"),
+}
+SEED = 20261204
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_inputs(name):
+    from ams_hip import gen
+    w = WORKLOADS[name]
+    if w["kind"] == "rmat":
+        abc = gen.G500 if w["abc"] == "G500" else gen.ER
+        a = gen.rmat_symmetric(w["scale"], w["ef"], abc, seed=SEED + w["scale"])
+        return a, a  # symmetric: A^T = A
+    inc = gen.join_incidence(seed=SEED)
+    return inc.transpose(), inc  # C = A^T A: csr_a = A^T, csr_b = A
+
+
+def alg_bytes(phase, st, v, m):
+    """Algorithmic HBM bytes per launch of one phase (two-pass Gustavson, no
+    reuse credited, 4-byte indices, v-byte values):
+      symbolic = 4 P + 12 nnz(A) + 16 (rows + 1)
+      numeric  = (4 + v) P + (12 + v) nnz(A) + (4 + v) nnz(C) + 16 (rows + 1)
+    restricted to the rows the phase processes."""
+    if phase in ("numc_dense", "numc_hash", "numc_esc"):
+        r = {"numc_hash": 0, "numc_esc": 1, "numc_dense": 2}[phase]
+        p, z, za = st["flops_route"][r], st["nnzc_route"][r], st["nnza_route"][r]
+        rows = [st["rows_num_hash"], st["rows_num_esc"], st["rows_num_dense"]][r]
+        return (4 + v) * p + (12 + v) * za + (4 + v) * z + 16 * (rows + 1)
+    if phase in ("symb_hash", "symb_bitmap"):
+        r = 0 if phase == "symb_hash" else 1
+        rows = st["rows_sym_hash"] if r == 0 else st["rows_sym_bitmap"]
+        return 4 * st["flops_sym"][r] + 12 * st["nnza_sym"][r] + 16 * (rows + 1)
+    return None
+
+
+def whole_bytes(st, v, nnz_a, m):
+    return (8 + v) * st["flops"] + (24 + v) * nnz_a + (4 + v) * st["nnz_c"] + 16 * (m + 1)
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration, from the committed summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}__{workload}.pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(a, b, target_s, threads):
+    """The oracle on every s-th row of A (same B), s chosen so the timed run
+    takes about target_s seconds on `threads` host threads."""
+    from ams_hip.csr import Csr
+    from oracle import oracle as orc
+
+    a64 = a.astype(np.int64, np.float64)
+    b64 = b if b is a else b.astype(np.int64, np.float64)
+    if b is a:
+        b64 = a64
+    flp = orc.flp_populate(a64, b64, threads)
+    total = int(flp.sum())
+
+    def sample(stride):
+        rows = np.arange(0, a64.num_row, stride)
+        cnt = np.diff(a64.row_ptrs)[rows]
+        rp = np.zeros(len(rows) + 1, np.int64)
+        np.cumsum(cnt, out=rp[1:])
+        idx = np.concatenate([np.arange(a64.row_ptrs[r], a64.row_ptrs[r + 1]) for r in rows]) \
+            if len(rows) else np.zeros(0, np.int64)
+        sub = Csr(len(rows), a64.num_col, rp, a64.cols[idx], a64.rats[idx])
+        return sub, int(flp[rows].sum())
+
+    p = orc.params(threads)
+    stride = max(1, int(total / 2e8))
+    sub, ps = sample(stride)
+    t0 = time.perf_counter()
+    orc.spgemm(sub, b64, "hybacc", canonical=False, p=p)
+    tc = time.perf_counter() - t0
+    rate = ps / max(tc, 1e-3)
+    want = rate * target_s
+    stride = max(1, int(total / max(want, 1.0)))
+    sub, ps = sample(stride)
+    t0 = time.perf_counter()
+    _, _, _, rt = orc.spgemm(sub, b64, "hybacc", canonical=False, p=p)
+    wall = time.perf_counter() - t0
+    t = rt["total_time"]
+    return {"value": 2.0 * ps / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th row of A ({sub.num_row} rows, {ps} products = "
+                      f"{100.0 * ps / total:.2f}% of P), same B; oracle total_time {t:.2f} s "
+                      f"(wall {wall:.2f} s) on {threads} threads; AMS_PRT 0.85, AMS_ACC_THRES 2.0, "
+                      f"AMS_MIN_NUM 32768, DN_MIN 256"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=9)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="g500_s19_ef32", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--param", action="append", default=[], help="routing parameter key=value")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is a design "
+              "(row blocks split by flops), not a run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from ams_hip.device import Context, DeviceCsr
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    a, b = make_inputs(args.workload)
+    log(f"inputs {args.workload}: A {a.num_row}x{a.num_col} nnz {a.num_nnz}, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    da = DeviceCsr.from_host(a)
+    db = da if b is a else DeviceCsr.from_host(b)
+    ctx = Context()
+    if args.param:
+        kv = {}
+        for s in args.param:
+            k, v = s.split("=", 1)
+            kv[k] = float(v) if "." in v else int(v)
+        ctx.set_params(**kv)
+    _, P = ctx.row_flops(da, db)
+    log(f"P = {P}")
+    for i in range(args.warmup):
+        ctx.spgemm_pooled(da, db)
+        torch.cuda.synchronize()
+        log(f"warmup {i} done")
+    ctx.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        c = ctx.spgemm_pooled(da, db)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tm = ctx.timings()
+    ctx.enable_timing(False)
+    st = ctx.stats()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = 2.0 * P / (ms_step * 1e-3) / 1e9
+    log(f"{args.steps} steps, {ms_step:.3f} ms/step, {value:.1f} GFLOP/s; phases {tm}")
+
+    v = 8
+    phases = {k: ms for k, (ms, n) in tm.items()}
+    dom = max(phases, key=phases.get)
+    ms_dom, n_dom = tm[dom]
+    bytes_dom = alg_bytes(dom, st, v, a.num_row)
+    roof = None
+    if bytes_dom is not None and n_dom > 0:
+        per_launch_s = ms_dom / n_dom * 1e-3
+        ach = bytes_dom / per_launch_s
+        kernel = {"numc_dense": "k_numc_dense", "symb_bitmap": "k_symb_bitmap", "numc_hash": "k_numc_hash",
+                  "symb_hash": "k_symb_hash", "numc_esc": "k_numc_esc"}[dom]
+        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": read_traffic(kernel, args.workload),
+                "kernel": kernel, "alg_bytes_per_launch": bytes_dom, "avg_launch_ms": ms_dom / n_dom}
+    wb = whole_bytes(st, v, a.num_nnz, a.num_row)
+
+    ver = ctx.verify(c, 0)
+    checks = {"nnz_c": ver["nnz"], "sum_col": ver["sum_col"], "sum_val": ver["sum_val"],
+              "unsorted_rows": ver["unsorted_rows"], "sum_val_equals_P": ver["sum_val"] == float(P)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads, target {args.cpu_seconds} s")
+        cpu = cpu_baseline(a, b, args.cpu_seconds, threads)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded R-MAT / incidence generator)",
+        "config": {"workload": WORKLOADS[args.workload]["text"], "name": args.workload, "m": a.num_row,
+                   "n": b.num_col, "nnz_a": a.num_nnz, "flops_P": P, "nnz_c": st["nnz_c"],
+                   "params": ctx.params()},
+        "roofline": roof,
+        "pipeline": {"alg_bytes_per_step": wb, "frac_hbm_roofline": wb / (ms_step * 1e-3) / HBM_PEAK,
+                     "phase_ms_per_step": {k: ms / args.steps for k, ms in phases.items()},
+                     "routes": {k: st[k] for k in ("rows_sym_hash", "rows_sym_bitmap", "rows_num_hash",
+                                                   "rows_num_esc", "rows_num_dense", "dense_chunks")}},
+        "checks": checks,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

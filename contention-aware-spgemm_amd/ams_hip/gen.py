@@ -1,0 +1,142 @@
+"""Seeded synthetic inputs for the SpGEMM workloads.
+
+The reference draws its R-MAT graphs with PaRMAT (PaRMAT/src/utils.cpp:63-172,
+GraphGen_sorted.cpp:17-207) and turns them into symmetric all-ones
+MatrixMarket pairs with rmat_ones (src/rmat_ones.c:6-37).  PaRMAT seeds from
+std::random_device and time(0), so its graphs cannot be reproduced; this
+module draws the same distribution from a fixed numpy seed instead:
+
+* ``rmat_symmetric``: R-MAT edges (quadrant probabilities a, b, c, d), self
+  loops dropped, undirected duplicates dropped, exactly ``ef * 2**scale``
+  distinct undirected edges kept in draw order, then mirrored.  nnz = 2 * ef *
+  2**scale, columns ascending within every row.
+* ``join_incidence``: the join-aggregate-shaped incidence matrix (lognormal
+  row degrees, Zipf column ids drawn without repeats inside a row).
+* ``batch_problems``: the uneven batch of small skewed R-MAT problems.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .csr import Csr
+
+G500 = (0.57, 0.19, 0.19)
+ER = (0.25, 0.25, 0.25)
+
+
+def _rmat_batch(rng, scale, count, a, b, c):
+    src = np.zeros(count, np.int64)
+    dst = np.zeros(count, np.int64)
+    ab, abc = a + b, a + b + c
+    for _ in range(scale):
+        u = rng.random(count, dtype=np.float64)
+        row_bit = u >= ab
+        col_bit = ((u >= a) & (u < ab)) | (u >= abc)
+        src <<= 1
+        dst <<= 1
+        src |= row_bit
+        dst |= col_bit
+    return src, dst
+
+
+def rmat_undirected_edges(scale, ef, abc=G500, seed=0):
+    """Exactly ef * 2**scale distinct undirected, loop-free edges (lo < hi)."""
+    a, b, c = abc
+    n = 1 << scale
+    target = ef * n
+    rng = np.random.default_rng(seed)
+    keys = np.empty(0, np.int64)
+    batch = target + target // 4 + 1024
+    for _ in range(64):
+        s, d = _rmat_batch(rng, scale, batch, a, b, c)
+        lo, hi = np.minimum(s, d), np.maximum(s, d)
+        keep = lo != hi
+        keys = np.concatenate([keys, lo[keep] * n + hi[keep]])
+        _, first = np.unique(keys, return_index=True)
+        if len(first) >= target:
+            first.sort()
+            keys = keys[first[:target]]
+            return keys // n, keys % n
+        keys = keys[np.sort(first)]
+        deficit = target - len(keys)
+        batch = 2 * deficit + 1024
+    raise RuntimeError("R-MAT generator did not reach the edge target")
+
+
+def rmat_symmetric(scale, ef, abc=G500, seed=0, values="ones", val_dtype=np.float64):
+    """Symmetric R-MAT adjacency as CSR (int64 row_ptrs, int32 cols).
+
+    values: "ones" (all 1.0), "uniform" (U[-1, 1], equal on both mirrored
+    entries) or "normal" (N(0, 1) drawn per stored entry, so A != A^T).
+    """
+    n = 1 << scale
+    lo, hi = rmat_undirected_edges(scale, ef, abc, seed)
+    rows = np.concatenate([lo, hi])
+    cols = np.concatenate([hi, lo])
+    vrng = np.random.default_rng(seed + 7919)
+    if values == "ones":
+        vals = np.ones(len(rows), val_dtype)
+    elif values == "uniform":
+        v = vrng.uniform(-1.0, 1.0, len(lo))
+        vals = np.concatenate([v, v]).astype(val_dtype)
+    elif values == "normal":
+        vals = vrng.standard_normal(len(rows)).astype(val_dtype)
+    else:
+        raise ValueError(values)
+    order = np.argsort(rows * n + cols, kind="stable")
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    return Csr(n, n, rp, cols.astype(np.int32), vals)
+
+
+def join_incidence(num_row=850_000, num_col=20_000, mu=2.0, sigma=1.5, cap=5000,
+                   zipf_s=1.2, seed=0):
+    """Rows: lognormal degrees clipped to [1, cap]; columns: Zipf(zipf_s) ids
+    (rank r -> column r - 1) without repeats inside a row; values 1.0."""
+    rng = np.random.default_rng(seed)
+    deg = np.clip(np.rint(rng.lognormal(mu, sigma, num_row)), 1, cap).astype(np.int64)
+    pmf = np.arange(1, num_col + 1, dtype=np.float64) ** (-zipf_s)
+    cdf = np.cumsum(pmf)
+    cdf /= cdf[-1]
+    rows = np.empty(0, np.int64)
+    cols = np.empty(0, np.int64)
+    need = deg.copy()
+    for _ in range(200):
+        todo = np.nonzero(need > 0)[0]
+        if len(todo) == 0:
+            break
+        draw = np.repeat(todo, need[todo] * 2 + 4)
+        c = np.searchsorted(cdf, rng.random(len(draw)), side="right")
+        c = np.minimum(c, num_col - 1)
+        key = np.concatenate([rows * num_col + cols, draw * num_col + c])
+        # keep the first occurrence of each (row, col), in draw order
+        _, first = np.unique(key, return_index=True)
+        first.sort()
+        key = key[first]
+        r, cc = key // num_col, key % num_col
+        # truncate every row to its degree, keeping draw order
+        order = np.argsort(r, kind="stable")
+        r, cc = r[order], cc[order]
+        start = np.searchsorted(r, np.arange(num_row))
+        rank = np.arange(len(r)) - start[r]
+        keep = rank < deg[r]
+        rows, cols = r[keep], cc[keep]
+        need = deg - np.bincount(rows, minlength=num_row)
+    else:
+        raise RuntimeError("join incidence generator did not converge")
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    rp = np.zeros(num_row + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=num_row), out=rp[1:])
+    return Csr(num_row, num_col, rp, cols.astype(np.int32), np.ones(len(rows), np.float64))
+
+
+def batch_problems(count=512, seed=0, scales=(8, 14), efs=(2, 4, 8, 16, 32)):
+    """The uneven batch: scale ~ U{scales}, ef ~ U{efs}, skewed R-MAT,
+    symmetric, values U[-1, 1]."""
+    rng = np.random.default_rng(seed)
+    sc = rng.integers(scales[0], scales[1] + 1, count)
+    ef = rng.choice(np.asarray(efs), count)
+    return [rmat_symmetric(int(s), int(e), G500, seed=seed * 100003 + i, values="uniform")
+            for i, (s, e) in enumerate(zip(sc, ef))]

@@ -1,0 +1,90 @@
+// ams_kernels.hpp -- the SpGEMM kernels (declarations of the templates are
+// instantiated in ams_spgemm.hip and ams_bwd.hip).
+//
+// Data layout in HBM (all device arrays):
+//   A, B, C   CSR: int64 row_ptr[m+1], int32 col[nnz], V val[nnz] (V = double|float)
+//   row_flp   int64[m]   products per row of A*B
+//   sym_bin   uint8[m]   symbolic route of each row, num_bin uint8[m] numeric route
+//   lists     int32      rows grouped by bin (one segment per bin)
+//   coff      int32[H*(R+1)]  per heavy row: exclusive prefix of nnz over the
+//             R column chunks of width W, coff[h*(R+1)+R] = nnz of the row
+//   bptr      int64[R*(mB+1)] chunk-major row pointers of the column-blocked
+//             copy of B (bcol: uint16 column within chunk, bval: V)
+//
+// Problem table: a stacked batch is one block-diagonal product.  Problem i
+// owns A rows [arow[i], arow[i+1]), B rows (= A columns) [brow[i], brow[i+1])
+// and output columns [ccol[i], ccol[i+1]).  A single product is nprob = 1.
+#pragma once
+
+#include "ams_device.hpp"
+
+namespace ams {
+
+// Symbolic routes.
+enum : uint8_t { SB_NONE = 0xff, SB_BITMAP = 8 };  // 0..7: LDS hash, table 64 << b
+constexpr int kSymHashBins = 8;
+constexpr int kSymBins = 9;
+// Numeric routes.
+enum : uint8_t { NB_NONE = 0xff, NB_DENSE = 16 };  // 0..7 hash (64 << b), 8..14 ESC (64 << (b-8))
+constexpr int kNumHashBins = 8;
+constexpr int kNumEscBins = 7;
+constexpr int kNumBins = 17;
+constexpr int kMaxHashSlots = 64 << (kNumHashBins - 1);  // 8192
+constexpr int kMaxEsc = 64 << (kNumEscBins - 1);         // 4096
+
+struct Csr {
+  int64_t m, n, nnz;
+  const int64_t* rp;
+  const int32_t* col;
+  const void* val;
+};
+
+struct Probs {
+  int32_t n;              // number of problems; n <= 1: one product, the
+                          // table pointers are unused
+  const int64_t* arow;    // n + 1
+  const int64_t* brow;    // n + 1
+  const int64_t* ccol;    // n + 1
+  int64_t width0;         // output columns when n <= 1
+  int64_t max_width;      // max output columns of one problem
+};
+
+__device__ __forceinline__ int find_prob(const int64_t* off, int n, int64_t r) {
+  if (n <= 1) return 0;
+  int lo = 0, hi = n;  // off[lo] <= r < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= r) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Column base and width of the problem that owns A row `row`.
+__device__ __forceinline__ void prob_cols(const Probs& pr, int64_t row, int64_t& lo, int64_t& width) {
+  if (pr.n <= 1) { lo = 0; width = pr.width0; return; }
+  const int pi = find_prob(pr.arow, pr.n, row);
+  lo = pr.ccol[pi];
+  width = pr.ccol[pi + 1] - lo;
+}
+
+// Column base of the problem that owns B row k.
+__device__ __forceinline__ int64_t prob_bcol_lo(const Probs& pr, int64_t k) {
+  if (pr.n <= 1) return 0;
+  return pr.ccol[find_prob(pr.brow, pr.n, k)];
+}
+
+struct SymParams {
+  int64_t sym_hash_max;
+  double bitmap_ratio;
+  int64_t bitmap_bits;  // usable LDS bitmap width (0 disables the bitmap route)
+};
+
+struct NumParams {
+  int64_t num_hash_max;
+  int64_t esc_max;
+  double acc_thres;
+  double dense_density;
+  int32_t mode;  // 0 hybrid, 1 hash, 2 esc
+};
+
+}  // namespace ams

@@ -1,0 +1,702 @@
+// ams_kernels_impl.hpp -- HIP/CDNA4 kernels of the hybrid-accumulation SpGEMM.
+//
+// Pipeline (one stream, no MFMA: the work is sparse gathers and LDS atomics):
+//   k_row_flops      row_flp[i] = sum_{k in A(i,:)} |B(k,:)|
+//                    (flp_populate, reference src/mm/mm_utils.c:1063-1125)
+//   k_classify_sym   symbolic route per row: LDS hash table sized by flops,
+//                    or the LDS bitmap for wide rows
+//   k_scatter_bins   rows grouped by route (replaces the morsel packing of
+//                    pack_ams_pack, src/ams/ams_utils.c:1410-1596)
+//   k_symb_hash      distinct columns per row with an LDS hash
+//                    (ams_hash_symb_dnr, src/ams/ams_utils.c:1600-1752)
+//   k_symb_bitmap    distinct columns per row and per dense column chunk
+//   k_scan_*         exclusive scan (scan, src/mm/mm_utils.c:866-957)
+//   k_classify_num   numeric route: ESC for low compression, LDS hash, or
+//                    dense column chunks (the cf routing of
+//                    ams_hybacc_numc_krnl, src/ams/ams_utils.c:1141-1149)
+//   k_numc_hash      LDS hash accumulate, sort keys, write sorted row
+//                    (hybacc_hash_krnl, src/ams/ams_utils.c:956-1019)
+//   k_numc_esc       expand to LDS, bitonic sort (key, rank), compress runs
+//                    left to right (hybacc_hsersc_krnl, :1023-1075)
+//   k_bblk_*         column-blocked copy of B for the dense bin
+//   k_numc_dense     persistent kernel: a global ticket queue hands out
+//                    (heavy row, column chunk) morsels; each morsel is
+//                    accumulated in a dense LDS array and written sorted
+#pragma once
+#include "ams_kernels.hpp"
+
+namespace ams {
+
+// ---------------------------------------------------------------- helpers
+
+__global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_i64_to_i32(const int64_t* in, int32_t* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (int32_t)in[i];
+}
+
+__global__ void k_i32_to_i64(const int32_t* in, int64_t* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (int64_t)in[i];
+}
+
+__device__ __forceinline__ int64_t next_pow2_i64(int64_t x) {
+  return x <= 1 ? 1 : (int64_t)1 << (64 - __clzll((unsigned long long)(x - 1)));
+}
+
+__device__ __forceinline__ int ilog2_pow2(int64_t x) { return 63 - __clzll((unsigned long long)x); }
+
+// Segment of A nonzero p in the original B: B row A.col[p].
+struct SegB {
+  const int32_t* acol;
+  const int64_t* brp;
+  __device__ __forceinline__ void operator()(int64_t p, int64_t& st, int32_t& len) const {
+    const int32_t k = acol[p];
+    st = brp[k];
+    len = (int32_t)(brp[k + 1] - st);
+  }
+};
+
+// ------------------------------------------------------------- row flops
+
+// One wave per row of A; also accumulates P into *total.
+__global__ __launch_bounds__(256) void k_row_flops(Csr A, Csr B, int64_t* row_flp,
+                                                   unsigned long long* total) {
+  __shared__ int64_t s_w[4];
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  int64_t acc = 0;
+  for (int64_t r = blockIdx.x * 4 + wave_id(); r < A.m; r += waves) {
+    const int64_t s = A.rp[r], e = A.rp[r + 1];
+    int64_t f = 0;
+    for (int64_t p = s + lane_id(); p < e; p += kWave) {
+      const int32_t k = A.col[p];
+      f += B.rp[k + 1] - B.rp[k];
+    }
+    f = wave_sum(f);
+    if (lane_id() == 0) {
+      if (row_flp) row_flp[r] = f;
+      acc += f;
+    }
+  }
+  const int64_t t = block_sum<256, int64_t>(acc, s_w);
+  if (threadIdx.x == 0 && t) atomicAdd(total, (unsigned long long)t);
+}
+
+// ------------------------------------------------------------ binning
+
+// Wave-aggregated append of this lane's row (bin b, b != 0xff) into the
+// per-bin counters; returns the rank within the bin.
+__device__ __forceinline__ int32_t wave_append(int32_t* counters, int b, bool active) {
+  int32_t rank = -1;
+  unsigned long long pending = __ballot(active);
+  while (pending) {
+    const int lead = __ffsll(pending) - 1;
+    const int bb = __shfl(b, lead, kWave);
+    const unsigned long long mask = __ballot(active && b == bb);
+    int32_t base = 0;
+    if (lane_id() == lead) base = atomicAdd(&counters[bb], (int32_t)__popcll(mask));
+    base = __shfl(base, lead, kWave);
+    if (active && b == bb) {
+      rank = base + (int32_t)__popcll(mask & ((1ull << lane_id()) - 1));
+      active = false;
+    }
+    pending &= ~mask;
+  }
+  return rank;
+}
+
+__device__ __forceinline__ int64_t row_width(const Probs& pr, int64_t row) {
+  int64_t lo, width;
+  prob_cols(pr, row, lo, width);
+  return width;
+}
+
+// Symbolic route of every row.  Rows without products get nnz 0 here.
+__global__ __launch_bounds__(256) void k_classify_sym(int64_t m, const int64_t* row_flp, Probs pr,
+                                                      SymParams sp, uint8_t* sbin, int64_t* rownz,
+                                                      int32_t* counts) {
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool valid = i < m;
+    int b = SB_NONE;
+    if (valid) {
+      const int64_t f = row_flp[i];
+      if (f == 0) {
+        rownz[i] = 0;
+      } else {
+        const int64_t words = (row_width(pr, i) + 31) >> 5;
+        if (f > sp.sym_hash_max || (double)f >= sp.bitmap_ratio * (double)words) {
+          b = SB_BITMAP;
+        } else {
+          const int64_t s = next_pow2_i64(f + ((f + 1) >> 1));
+          b = s <= 64 ? 0 : ilog2_pow2(s) - 6;
+        }
+      }
+      sbin[i] = (uint8_t)b;
+    }
+    (void)wave_append(counts, b, b != SB_NONE);
+  }
+}
+
+// Numeric route of every row (needs C's row pointers).
+__global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* row_flp,
+                                                      const int64_t* rp_c, const uint8_t* sbin,
+                                                      Probs pr, NumParams np, uint8_t* nbin,
+                                                      int32_t* counts) {
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool valid = i < m;
+    int b = NB_NONE;
+    if (valid) {
+      const int64_t f = row_flp[i];
+      const int64_t z = rp_c[i + 1] - rp_c[i];
+      if (f > 0 && z > 0) {
+        const bool bitmap = sbin[i] == SB_BITMAP;
+        const double width = (double)row_width(pr, i);
+        if (bitmap && (z > np.num_hash_max || (double)z >= np.dense_density * width)) {
+          b = NB_DENSE;
+        } else if (np.mode != 1 && f <= np.esc_max &&
+                   (np.mode == 2 || (double)f / (double)z < np.acc_thres)) {
+          const int64_t e = next_pow2_i64(f);
+          b = kNumHashBins + (e <= 64 ? 0 : ilog2_pow2(e) - 6);
+        } else if (z <= np.num_hash_max) {
+          const int64_t s = next_pow2_i64(z + ((z + 1) >> 1));
+          b = s <= 64 ? 0 : ilog2_pow2(s) - 6;
+        } else {
+          b = NB_DENSE;
+        }
+      }
+      nbin[i] = (uint8_t)b;
+    }
+    (void)wave_append(counts, b, b != NB_NONE);
+  }
+}
+
+// Rows grouped by bin: bin b occupies list[sum(counts[0..b)) ...).
+__global__ __launch_bounds__(256) void k_scatter_bins(int64_t m, const uint8_t* bin, int nbins,
+                                                      const int32_t* counts, int32_t* cursors,
+                                                      int32_t* list) {
+  __shared__ int32_t s_off[32];
+  if (threadIdx.x == 0) {
+    int32_t acc = 0;
+    for (int b = 0; b < nbins; ++b) { s_off[b] = acc; acc += counts[b]; }
+  }
+  __syncthreads();
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const int b = i < m ? (int)bin[i] : 0xff;
+    const bool active = b != 0xff;
+    const int32_t r = wave_append(cursors, b, active);
+    if (active) list[s_off[b] + r] = (int32_t)i;
+  }
+}
+
+__device__ __forceinline__ void bin_range(const int32_t* counts, int bin, int32_t& start, int32_t& cnt) {
+  int32_t s = 0;
+  for (int b = 0; b < bin; ++b) s += counts[b];
+  start = s;
+  cnt = counts[bin];
+}
+
+// ----------------------------------------------------------------- scan
+
+constexpr int kScanT = 256;
+constexpr int kScanIpt = 8;
+constexpr int kScanTile = kScanT * kScanIpt;
+
+__global__ __launch_bounds__(kScanT) void k_scan_tiles(const int64_t* in, int64_t n, int64_t* tsum) {
+  __shared__ int64_t s_w[kScanT / kWave];
+  const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanIpt;
+  int64_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanIpt; ++j) acc += (base + j < n) ? in[base + j] : 0;
+  const int64_t t = block_sum<kScanT, int64_t>(acc, s_w);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int64_t* tsum, int64_t nt) {
+  __shared__ int64_t s_w[16];
+  int64_t carry = 0;
+  for (int64_t b = 0; b < nt; b += 1024) {
+    const int64_t i = b + threadIdx.x;
+    const int64_t v = i < nt ? tsum[i] : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan<1024, int64_t>(v, s_w, tot);
+    if (i < nt) tsum[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(kScanT) void k_scan_apply(const int64_t* in, int64_t* out, int64_t n,
+                                                       const int64_t* tsum) {
+  __shared__ int64_t s_w[kScanT / kWave];
+  const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanIpt;
+  int64_t v[kScanIpt];
+  int64_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanIpt; ++j) {
+    v[j] = (base + j < n) ? in[base + j] : 0;
+    acc += v[j];
+  }
+  int64_t tot;
+  int64_t run = block_excl_scan<kScanT, int64_t>(acc, s_w, tot) + tsum[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanIpt; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+}
+
+// ------------------------------------------------------------- symbolic
+
+// LDS-hash symbolic, one row per block; table of S int32 keys.
+template <int S, int T>
+__global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* list,
+                                                 const int32_t* counts, int bin, int64_t* rownz,
+                                                 int32_t* err) {
+  __shared__ int32_t keys[S];
+  __shared__ ExpandLds<T> L;
+  __shared__ int64_t s_w[T / kWave];
+  constexpr int shift = 32 - __builtin_ctz(S);
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const SegB seg{A.col, B.rp};
+  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+    const int32_t row = list[start + idx];
+    for (int s = threadIdx.x; s < S; s += T) keys[s] = kEmpty;
+    __syncthreads();
+    int64_t mine = 0;
+    block_expand<T>(A.rp[row], A.rp[row + 1], L, seg,
+                    [&](int, int64_t, int64_t q, int64_t) {
+                      const int r = lds_insert_key(keys, shift, B.col[q]);
+                      if (r > 0) ++mine;
+                      if (r < 0) atomicOr(err, 8);
+                    });
+    const int64_t tot = block_sum<T, int64_t>(mine, s_w);
+    if (threadIdx.x == 0) rownz[row] = tot;
+  }
+}
+
+// LDS-bitmap symbolic for wide rows, one row per block.  The row's output
+// columns [0, width) are covered in passes of Wb bits; the population of
+// every W-column chunk goes to coff (exclusive prefix, total at index R).
+// Dynamic LDS: Wb/32 words of bitmap + Wb/W chunk counters.
+template <int T>
+__global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int32_t* list,
+                                                   const int32_t* counts, int bin, int64_t* rownz,
+                                                   int32_t* coff, int R, int W, int Wb) {
+  extern __shared__ uint32_t dyn[];
+  uint32_t* bm = dyn;
+  int32_t* ccount = (int32_t*)(dyn + (Wb >> 5));
+  __shared__ ExpandLds<T> L;
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const SegB seg{A.col, B.rp};
+  const int wchunk = W >> 5;
+  for (int32_t h = blockIdx.x; h < cnt; h += gridDim.x) {
+    const int32_t row = list[start + h];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    int32_t running = 0;
+    int32_t* crow = coff + (int64_t)h * (R + 1);
+    for (int64_t c0 = 0; c0 < width; c0 += Wb) {
+      const int32_t pw = (int32_t)((width - c0) < Wb ? (width - c0) : Wb);
+      const int32_t words = (pw + 31) >> 5;
+      const int32_t nch = (pw + W - 1) / W;
+      for (int w = threadIdx.x; w < words; w += T) bm[w] = 0u;
+      for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
+      __syncthreads();
+      const int64_t base = lo + c0;
+      block_expand<T>(A.rp[row], A.rp[row + 1], L, seg,
+                      [&](int, int64_t, int64_t q, int64_t) {
+                        const uint32_t u = (uint32_t)((int64_t)B.col[q] - base);
+                        if (u < (uint32_t)pw) atomicOr(&bm[u >> 5], 1u << (u & 31));
+                      });
+      for (int w = threadIdx.x; w < words; w += T) {
+        const int pc = __popc(bm[w]);
+        if (pc) atomicAdd(&ccount[w / wchunk], pc);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const int c_first = (int)(c0 / W);
+        for (int c = 0; c < nch; ++c) {
+          crow[c_first + c] = running;
+          running += ccount[c];
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const int rrow = (int)((width + W - 1) / W);
+      for (int c = rrow; c <= R; ++c) crow[c] = running;
+      rownz[row] = running;
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------------- numeric
+
+// LDS hash accumulate of one row per block (S slots of int32 key + V),
+// then the keys are sorted and each is looked up again for its value, so
+// the row is written with ascending columns in coalesced stores.
+template <int S, int T, typename V>
+__global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* list,
+                                                 const int32_t* counts, int bin, const int64_t* rp_c,
+                                                 int32_t* col_c, V* val_c, int64_t cap, int32_t* err) {
+  __shared__ int32_t keys[S];
+  __shared__ V vals[S];
+  __shared__ int32_t sk[S];
+  __shared__ ExpandLds<T> L;
+  __shared__ V aval[T];
+  __shared__ int32_t s_n;
+  constexpr int shift = 32 - __builtin_ctz(S);
+  if (rp_c[A.m] > cap) return;
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const V* av = (const V*)A.val;
+  const V* bv = (const V*)B.val;
+  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+    const int32_t row = list[start + idx];
+    const int64_t off = rp_c[row];
+    const int32_t z = (int32_t)(rp_c[row + 1] - off);
+    for (int s = threadIdx.x; s < S; s += T) { keys[s] = kEmpty; vals[s] = V(0); }
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    block_expand<T>(A.rp[row], A.rp[row + 1], L,
+                    [&](int64_t p, int64_t& st, int32_t& len) {
+                      const int32_t k = A.col[p];
+                      st = B.rp[k];
+                      len = (int32_t)(B.rp[k + 1] - st);
+                      aval[threadIdx.x] = av[p];
+                    },
+                    [&](int j, int64_t, int64_t q, int64_t) {
+                      if (!lds_insert_add<V>(keys, vals, shift, B.col[q], aval[j] * bv[q]))
+                        atomicOr(err, 16);
+                    });
+    for (int s0 = 0; s0 < S; s0 += T) {
+      const int s = s0 + threadIdx.x;
+      const bool occ = s < S && keys[s] != kEmpty;
+      const unsigned long long mask = __ballot(occ);
+      int32_t base = 0;
+      if (lane_id() == 0 && mask) base = atomicAdd(&s_n, (int32_t)__popcll(mask));
+      base = __shfl(base, 0, kWave);
+      if (occ) sk[base + __popcll(mask & ((1ull << lane_id()) - 1))] = keys[s];
+    }
+    __syncthreads();
+    const int32_t n = (int32_t)next_pow2_i64(z);
+    if (threadIdx.x == 0 && s_n != z) atomicOr(err, 1);
+    const int32_t zz = s_n < z ? s_n : z;  // never sort unset keys
+    for (int i = zz + threadIdx.x; i < n; i += T) sk[i] = 0x7fffffff;
+    __syncthreads();
+    block_bitonic_sort<T, int32_t>(sk, n);
+    for (int r = threadIdx.x; r < z; r += T) {
+      const int32_t key = sk[r];
+      const uint32_t slot = lds_find(keys, shift, key);
+      col_c[off + r] = key;
+      val_c[off + r] = slot < (uint32_t)S ? vals[slot] : V(0);
+    }
+    __syncthreads();
+  }
+}
+
+// ESC of one row per block: expand the row's f <= E products into LDS as
+// (column << 32 | rank) with their values, bitonic-sort, then every run of
+// equal columns is summed left to right by the thread that owns its head.
+// The rank keeps equal columns in A-row order, so each value is the same
+// left-to-right sum of separately rounded products as the reference's.
+template <int E, int T, typename V>
+__global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* list,
+                                                const int32_t* counts, int bin, const int64_t* row_flp,
+                                                const int64_t* rp_c, int32_t* col_c, V* val_c,
+                                                int64_t cap, int32_t* err) {
+  __shared__ uint64_t sk[E];
+  __shared__ V vals[E];
+  __shared__ ExpandLds<T> L;
+  __shared__ V aval[T];
+  __shared__ int64_t s_w[T / kWave];
+  if (rp_c[A.m] > cap) return;
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const V* av = (const V*)A.val;
+  const V* bv = (const V*)B.val;
+  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+    const int32_t row = list[start + idx];
+    const int32_t f = (int32_t)row_flp[row];
+    const int64_t off = rp_c[row];
+    const int32_t z = (int32_t)(rp_c[row + 1] - off);
+    block_expand<T>(A.rp[row], A.rp[row + 1], L,
+                    [&](int64_t p, int64_t& st, int32_t& len) {
+                      const int32_t k = A.col[p];
+                      st = B.rp[k];
+                      len = (int32_t)(B.rp[k + 1] - st);
+                      aval[threadIdx.x] = av[p];
+                    },
+                    [&](int j, int64_t, int64_t q, int64_t t) {
+                      sk[t] = ((uint64_t)(uint32_t)B.col[q] << 32) | (uint64_t)t;
+                      vals[t] = aval[j] * bv[q];
+                    });
+    const int32_t n = (int32_t)next_pow2_i64(f);
+    for (int i = f + threadIdx.x; i < n; i += T) sk[i] = ~0ull;
+    __syncthreads();
+    block_bitonic_sort<T, uint64_t>(sk, n);
+    const int32_t ipt = n >= T ? n / T : 1;
+    const int32_t e0 = threadIdx.x * ipt;
+    const int32_t e1 = (e0 + ipt) < f ? (e0 + ipt) : f;
+    int64_t heads = 0;
+    for (int32_t e = e0; e < e1; ++e)
+      heads += (e == 0) || ((uint32_t)(sk[e] >> 32) != (uint32_t)(sk[e - 1] >> 32));
+    int64_t tot;
+    int64_t rank = block_excl_scan<T, int64_t>(heads, s_w, tot);
+    if (threadIdx.x == 0 && tot != z) atomicOr(err, 2);
+    for (int32_t e = e0; e < e1; ++e) {
+      const uint32_t key = (uint32_t)(sk[e] >> 32);
+      if (e != 0 && key == (uint32_t)(sk[e - 1] >> 32)) continue;
+      V sum = vals[(uint32_t)sk[e]];
+      for (int32_t g = e + 1; g < f && (uint32_t)(sk[g] >> 32) == key; ++g) sum += vals[(uint32_t)sk[g]];
+      col_c[off + rank] = (int32_t)key;
+      val_c[off + rank] = sum;
+      ++rank;
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------- column-blocked B
+
+// Entries per (chunk, B row): cnt[c*(mB+1) + k].  One wave per B row.
+// Chunk of an entry = (column - problem column base) / W.
+__global__ __launch_bounds__(64) void k_bblk_count(Csr B, Probs pr, int W, int R, int64_t* cnt) {
+  extern __shared__ int32_t cw[];
+  const int64_t ld = B.m + 1;
+  for (int64_t k = blockIdx.x; k < B.m; k += gridDim.x) {
+    const int64_t lo = prob_bcol_lo(pr, k);
+    for (int c = threadIdx.x; c < R; c += 64) cw[c] = 0;
+    __syncthreads();
+    for (int64_t q = B.rp[k] + threadIdx.x; q < B.rp[k + 1]; q += 64)
+      atomicAdd(&cw[(int)(((int64_t)B.col[q] - lo) / W)], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < R; c += 64) cnt[c * ld + k] = cw[c];
+    __syncthreads();
+  }
+}
+
+// Scatter B into the blocked layout; entries of one (chunk, row) segment
+// keep B's order.  perm (optional) records each entry's original position.
+template <typename V>
+__global__ __launch_bounds__(64) void k_bblk_scatter(Csr B, Probs pr, int W, int R, const int64_t* bptr,
+                                                     uint16_t* bcol, V* bval, int64_t* perm) {
+  extern __shared__ int32_t cw[];
+  const int64_t ld = B.m + 1;
+  const V* bv = (const V*)B.val;
+  for (int64_t k = blockIdx.x; k < B.m; k += gridDim.x) {
+    const int64_t lo = prob_bcol_lo(pr, k);
+    for (int c = threadIdx.x; c < R; c += 64) cw[c] = 0;
+    __syncthreads();
+    for (int64_t q0 = B.rp[k]; q0 < B.rp[k + 1]; q0 += 64) {
+      const int64_t q = q0 + threadIdx.x;
+      bool active = q < B.rp[k + 1];
+      int32_t loc = 0;
+      int c = 0;
+      if (active) {
+        loc = (int32_t)((int64_t)B.col[q] - lo);
+        c = loc / W;
+      }
+      unsigned long long pending = __ballot(active);
+      while (pending) {
+        const int lead = __ffsll(pending) - 1;
+        const int cc = __shfl(c, lead, kWave);
+        const unsigned long long mask = __ballot(active && c == cc);
+        const int32_t prev = cw[cc];
+        if (active && c == cc) {
+          const int64_t pos = bptr[cc * ld + k] + prev + __popcll(mask & ((1ull << threadIdx.x) - 1));
+          bcol[pos] = (uint16_t)(loc - cc * W);
+          bval[pos] = bv[q];
+          if (perm) perm[pos] = q;
+          active = false;
+        }
+        __syncthreads();
+        if (threadIdx.x == lead) cw[cc] = prev + (int32_t)__popcll(mask);
+        __syncthreads();
+        pending &= ~mask;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Dense-chunk numeric: persistent blocks take tickets t = c * H + h
+// (chunk-major, so concurrently running morsels read the same column block
+// of B) for heavy row h of the bitmap list and chunk c.  Dynamic LDS:
+//   V dense[W] | uint32 bitmap[W/32] | union{ExpandLds, uint16 idx[W]}
+template <int T, typename V>
+__global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64_t* bptr,
+                                                  const uint16_t* bcol, const V* bval,
+                                                  const int32_t* list, const int32_t* counts, int bin,
+                                                  const uint8_t* nbin, const int32_t* coff, int R, int W,
+                                                  const int64_t* rp_c, int32_t* col_c, V* val_c,
+                                                  int64_t cap, Probs pr, unsigned long long* ticket,
+                                                  int32_t* err) {
+  extern __shared__ __align__(16) unsigned char dyn_raw[];
+  V* dense = (V*)dyn_raw;
+  uint32_t* bm = (uint32_t*)(dense + W);
+  unsigned char* uni = (unsigned char*)(bm + (W >> 5));
+  ExpandLds<T>& L = *(ExpandLds<T>*)uni;
+  uint16_t* sel = (uint16_t*)uni;
+  __shared__ V aval[T];
+  __shared__ int64_t s_w[T / kWave];
+  __shared__ unsigned long long s_ticket;
+  if (rp_c[A.m] > cap) return;
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  const int words = W >> 5;
+  for (int i = threadIdx.x; i < W; i += T) dense[i] = V(0);
+  for (int i = threadIdx.x; i < words; i += T) bm[i] = 0u;
+  const unsigned long long ntask = (unsigned long long)H * (unsigned long long)R;
+  const V* av = (const V*)A.val;
+  const int64_t ld = mB + 1;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1ull);
+    __syncthreads();
+    const unsigned long long tk = s_ticket;
+    if (tk >= ntask) break;
+    const int c = (int)(tk / (unsigned long long)H);
+    const int32_t h = (int32_t)(tk - (unsigned long long)c * H);
+    const int32_t row = list[start + h];
+    if (nbin[row] != NB_DENSE) continue;
+    const int32_t* crow = coff + (int64_t)h * (R + 1);
+    const int32_t cbase = crow[c];
+    const int32_t ccnt = crow[c + 1] - cbase;
+    if (ccnt == 0) continue;
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    const int64_t off = rp_c[row] + cbase;
+    const int64_t* bp = bptr + (int64_t)c * ld;
+    block_expand<T>(A.rp[row], A.rp[row + 1], L,
+                    [&](int64_t p, int64_t& st, int32_t& len) {
+                      const int32_t k = A.col[p];
+                      st = bp[k];
+                      len = (int32_t)(bp[k + 1] - st);
+                      aval[threadIdx.x] = av[p];
+                    },
+                    [&](int j, int64_t, int64_t q, int64_t) {
+                      const uint32_t u = bcol[q];
+                      atomicOr(&bm[u >> 5], 1u << (u & 31));
+                      atomicAdd(&dense[u], aval[j] * bval[q]);
+                    });
+    // rank the set bits: each thread owns a run of consecutive words
+    const int wpt = (words + T - 1) / T;
+    const int w0 = threadIdx.x * wpt;
+    const int w1 = (w0 + wpt) < words ? (w0 + wpt) : words;
+    int64_t pc = 0;
+    for (int w = w0; w < w1; ++w) pc += __popc(bm[w]);
+    int64_t tot;
+    int32_t r = (int32_t)block_excl_scan<T, int64_t>(pc, s_w, tot);
+    if (threadIdx.x == 0 && tot != ccnt) atomicOr(err, 4);
+    for (int w = w0; w < w1; ++w) {
+      uint32_t x = bm[w];
+      bm[w] = 0u;
+      while (x) {
+        const int b = __ffs(x) - 1;
+        sel[r++] = (uint16_t)((w << 5) + b);
+        x &= x - 1;
+      }
+    }
+    __syncthreads();
+    const int32_t colbase = (int32_t)(lo + (int64_t)c * W);
+    const int32_t nout = (int32_t)tot < ccnt ? (int32_t)tot : ccnt;
+    for (int i = threadIdx.x; i < nout; i += T) {
+      const uint32_t u = sel[i];
+      col_c[off + i] = colbase + (int32_t)u;
+      val_c[off + i] = dense[u];
+      dense[u] = V(0);
+    }
+  }
+}
+
+}  // namespace ams
+
+namespace ams {
+
+__global__ void k_copy_i64(const int64_t* src, int64_t* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+// Work per route, for reports and per-kernel rooflines:
+//   sums[r]     products of numeric route r (0 hash, 1 ESC, 2 dense)
+//   sums[3 + r] nnz(C) of route r, sums[6 + r] nnz(A) of route r
+//   sums[9 + s] products of symbolic route s (0 hash, 1 bitmap)
+//   sums[11 + s] nnz(A) of symbolic route s
+constexpr int kRouteSums = 13;
+__global__ __launch_bounds__(256) void k_route_sums(int64_t m, const int64_t* row_flp, const int64_t* rp_c,
+                                                    const int64_t* rp_a, const uint8_t* sbin,
+                                                    const uint8_t* nbin, unsigned long long* sums) {
+  int64_t acc[kRouteSums];
+#pragma unroll
+  for (int i = 0; i < kRouteSums; ++i) acc[i] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = row_flp[i], z = rp_c[i + 1] - rp_c[i], d = rp_a[i + 1] - rp_a[i];
+    const int nb = nbin[i], sb = sbin[i];
+    const int r = nb == NB_NONE ? -1 : (nb == NB_DENSE ? 2 : (nb >= kNumHashBins ? 1 : 0));
+    if (r >= 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        acc[k] += (r == k) ? f : 0;
+        acc[3 + k] += (r == k) ? z : 0;
+        acc[6 + k] += (r == k) ? d : 0;
+      }
+    }
+    if (sb != SB_NONE) {
+      const int q = sb == SB_BITMAP ? 1 : 0;
+      acc[9 + q] += f;
+      acc[11 + q] += d;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kRouteSums; ++k) {
+    const int64_t v = wave_sum(acc[k]);
+    if (lane_id() == 0 && v) atomicAdd(&sums[k], (unsigned long long)v);
+  }
+}
+
+// Checksums of C: out[0] += sum of columns, out[1] += rows whose columns
+// are not strictly ascending; vsum += sum of values.
+template <typename V>
+__global__ __launch_bounds__(256) void k_verify(Csr C, unsigned long long* out, double* vsum) {
+  int64_t scol = 0, bad = 0;
+  double sval = 0.0;
+  const V* cv = (const V*)C.val;
+  for (int64_t r = blockIdx.x * 4 + wave_id(); r < C.m; r += (int64_t)gridDim.x * 4) {
+    const int64_t s = C.rp[r], e = C.rp[r + 1];
+    int64_t unsorted = 0;
+    for (int64_t q = s + lane_id(); q < e; q += kWave) {
+      scol += C.col[q];
+      sval += (double)cv[q];
+      if (q > s && C.col[q] <= C.col[q - 1]) unsorted = 1;
+    }
+    bad += wave_sum(unsorted) > 0 ? 1 : 0;
+  }
+  scol = wave_sum(scol);
+  sval = wave_sum(sval);
+  if (lane_id() == 0) {
+    atomicAdd(&out[0], (unsigned long long)scol);
+    if (bad) atomicAdd(&out[1], (unsigned long long)bad);
+    atomicAdd(vsum, sval);
+  }
+}
+
+}  // namespace ams

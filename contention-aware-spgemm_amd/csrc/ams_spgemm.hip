@@ -1,0 +1,999 @@
+// ams_spgemm.hip -- host side of the device API (include/ams_hip.h): the
+// context, scratch management, routing and launch of every phase.
+//
+// Plain product, one stream:
+//   symbolic: flops -> classify -> bins -> [1 sync: bin sizes] -> hash /
+//             bitmap counts -> scan -> numeric classify -> [1 sync: nnz]
+//   numeric:  [column-blocked B + dense queue kernel] + hash bins + ESC bins
+// The batched form runs the same kernels with grids sized from upper
+// bounds, so no host synchronisation is needed after its plan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ams_hip.h"
+#include "ams_kernels_impl.hpp"
+
+using namespace ams;
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+// Control block in device memory (zeroed by a fill kernel, never memset,
+// so that the sequence can be captured into a hipGraph).
+struct Ctrl {
+  int32_t cnt_sym[32];
+  int32_t cur_sym[32];
+  int32_t cnt_num[32];
+  int32_t cur_num[32];
+  int32_t err;
+  int32_t pad;
+  unsigned long long flops;
+  unsigned long long ticket;
+  unsigned long long sums[16];
+  int64_t pad2[5];
+};
+static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
+
+constexpr int kSymBitmapT = 512;
+constexpr int kDenseT = 512;
+
+template <int S>
+constexpr int hash_threads() { return S / 8 < 64 ? 64 : (S / 8 > 512 ? 512 : S / 8); }
+
+}  // namespace
+
+struct ams_ctx {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cu = 256;
+  size_t lds_max = 65536;
+  ams_params_t prm{};
+  ams_stats_t stats{};
+  std::string err;
+  Buf row_flp, sbin, nbin, lsym, lnum, ctrl, coff, bptr, bcol, bval, bperm, tsum;
+  Buf pool_rp, pool_col, pool_val, probs, scratch_a;
+  Ctrl* h_ctrl = nullptr;       // pinned mirror of the control block
+  int64_t* h_pin = nullptr;     // pinned scratch
+  // plan of the last symbolic call
+  bool plan_valid = false;
+  bool plan_counts = false;     // bin sizes known on the host
+  int64_t plan_m = 0, plan_mB = 0, plan_nnzB = 0, plan_H = 0;
+  int plan_R = 0, plan_W = 0, plan_Wb = 0, plan_dtype = 0;
+  Probs plan_pr{};
+  int32_t cnt_sym[kSymBins] = {}, cnt_num[kNumBins] = {};
+  // event timing of phases
+  bool timing = false;
+  std::vector<hipEvent_t> ev_free;
+  struct Rec { int ph; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  double acc_ms[AMS_PH_COUNT] = {};
+  int64_t acc_n[AMS_PH_COUNT] = {};
+  // batched plan
+  bool batch_ready = false;
+  Probs batch_pr{};
+  int64_t batch_m = 0, batch_mB = 0, batch_nnzA = 0, batch_nnzB = 0;
+  int batch_dtype = 0;
+};
+
+static int fail(ams_ctx_t* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ctx, AMS_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                             \
+  } while (0)
+
+#define CHK(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != AMS_OK) return r_; \
+  } while (0)
+
+static int grow(ams_ctx_t* ctx, Buf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.cap >= bytes) return AMS_OK;
+  if (b.p) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  const size_t want = bytes + bytes / 8;
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      b.p = nullptr;
+      return fail(ctx, AMS_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
+    }
+    b.cap = bytes;
+  } else {
+    b.cap = want;
+  }
+  return AMS_OK;
+}
+
+static Csr to_csr(const ams_dcsr_t* m) {
+  return Csr{m->num_row, m->num_col, m->num_nnz, m->row_ptr, m->col, m->val};
+}
+
+static int check_csr(ams_ctx_t* ctx, const ams_dcsr_t* m, const char* name) {
+  if (!m) return fail(ctx, AMS_ERR_ARG, "%s is NULL", name);
+  if (m->num_row < 0 || m->num_col < 0 || m->num_nnz < 0)
+    return fail(ctx, AMS_ERR_ARG, "%s has a negative size", name);
+  if (m->num_row >= INT32_MAX || m->num_col >= INT32_MAX)
+    return fail(ctx, AMS_ERR_LIMIT, "%s: rows and columns must be < 2^31", name);
+  if (!m->row_ptr) return fail(ctx, AMS_ERR_ARG, "%s has no row pointers", name);
+  if (m->num_nnz > 0 && (!m->col || !m->val)) return fail(ctx, AMS_ERR_ARG, "%s has no entries", name);
+  return AMS_OK;
+}
+
+template <typename K, typename... Args>
+static int occupancy(K kernel, int threads, size_t dyn, int* out) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, dyn) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 1;
+  }
+  *out = n < 1 ? 1 : n;
+  return AMS_OK;
+}
+
+static int grid_for(ams_ctx_t* ctx, int occ, int64_t work, bool known) {
+  int64_t g = (int64_t)ctx->num_cu * occ;
+  if (known) g = std::min<int64_t>(g, work);
+  return (int)std::max<int64_t>(g, 1);
+}
+
+static int scan_i64(ams_ctx_t* ctx, const int64_t* in, int64_t* out, int64_t n) {
+  if (n <= 0) return AMS_OK;
+  const int64_t nt = (n + kScanTile - 1) / kScanTile;
+  CHK(grow(ctx, ctx->tsum, sizeof(int64_t) * nt));
+  int64_t* ts = (int64_t*)ctx->tsum.p;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(kScanT), 0, ctx->stream, in, n, ts);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, ctx->stream, ts, nt);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kScanT), 0, ctx->stream, in, out, n,
+                     (const int64_t*)ts);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+static int fill_i64(ams_ctx_t* ctx, int64_t* p, int64_t n, int64_t v) {
+  if (n <= 0) return AMS_OK;
+  const int g = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i64, dim3(g), dim3(256), 0, ctx->stream, p, n, v);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+static int fill_i32(ams_ctx_t* ctx, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return AMS_OK;
+  const int g = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, ctx->stream, p, n, v);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+static Ctrl* dctrl(ams_ctx_t* ctx) { return (Ctrl*)ctx->ctrl.p; }
+
+static hipEvent_t take_event(ams_ctx_t* ctx) {
+  if (!ctx->ev_free.empty()) {
+    hipEvent_t e = ctx->ev_free.back();
+    ctx->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+// Brackets the launches of one phase with events when timing is on.
+struct Phase {
+  ams_ctx_t* ctx;
+  int ph;
+  hipEvent_t a = nullptr;
+  Phase(ams_ctx_t* c, int p, bool on) : ctx(c), ph(p) {
+    if (on && ctx->timing) {
+      a = take_event(ctx);
+      if (a) (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~Phase() {
+    if (!a) return;
+    hipEvent_t b = take_event(ctx);
+    if (!b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->recs.push_back({ph, a, b});
+  }
+};
+
+// Geometry of the dense column chunks and the symbolic bitmap passes.
+static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W, int& R, int& Wb) {
+  W = ctx->prm.dense_width;
+  if (max_width > 0 && W > max_width) W = (int)((max_width + 63) / 64 * 64);
+  W = std::max(64, std::min(W, 65536));
+  R = (int)((max_width + W - 1) / W);
+  if (R < 1) R = 1;
+  int64_t wb = (int64_t)ctx->prm.bitmap_bits / W * W;
+  if (wb < W) wb = W;
+  Wb = (int)wb;
+  (void)dtype;
+}
+
+static size_t dense_lds(int W, int vbytes) {
+  const size_t uni = std::max(sizeof(ExpandLds<kDenseT>), (size_t)W * 2);
+  return (size_t)W * vbytes + (size_t)(W / 32) * 4 + ((uni + 15) & ~(size_t)15);
+}
+
+// ------------------------------------------------------------ symbolic
+
+template <int S>
+static void launch_symb_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t* rownz, int bin) {
+  constexpr int T = hash_threads<S>();
+  const int64_t cnt = ctx->plan_counts ? ctx->cnt_sym[bin] : A.m;
+  if (ctx->plan_counts && cnt == 0) return;
+  int occ;
+  occupancy(k_symb_hash<S, T>, T, 0, &occ);
+  const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+  Ctrl* c = dctrl(ctx);
+  hipLaunchKernelGGL((k_symb_hash<S, T>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                     (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, bin, rownz, &c->err);
+}
+
+static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype,
+                        const Probs& pr, int64_t* rp_c, bool sync, int64_t* nnz) {
+  const Csr A = to_csr(a), B = to_csr(b);
+  const int64_t m = A.m;
+  ctx->plan_valid = false;
+  CHK(grow(ctx, ctx->ctrl, sizeof(Ctrl)));
+  CHK(grow(ctx, ctx->row_flp, sizeof(int64_t) * std::max<int64_t>(m, 1)));
+  CHK(grow(ctx, ctx->sbin, std::max<int64_t>(m, 1)));
+  CHK(grow(ctx, ctx->nbin, std::max<int64_t>(m, 1)));
+  CHK(grow(ctx, ctx->lsym, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  CHK(grow(ctx, ctx->lnum, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  Ctrl* c = dctrl(ctx);
+  CHK(fill_i32(ctx, (int32_t*)c, sizeof(Ctrl) / 4, 0));
+
+  int W, R, Wb;
+  dense_geometry(ctx, dtype, pr.max_width, W, R, Wb);
+  ctx->plan_W = W;
+  ctx->plan_R = R;
+  ctx->plan_Wb = Wb;
+  ctx->plan_m = m;
+  ctx->plan_mB = B.m;
+  ctx->plan_nnzB = B.nnz;
+  ctx->plan_dtype = dtype;
+  ctx->plan_pr = pr;
+
+  SymParams sp{ctx->prm.sym_hash_max, ctx->prm.bitmap_ratio, Wb};
+  if (m > 0) {
+    {
+      Phase ph(ctx, AMS_PH_FLOPS, sync);
+      const int g = (int)std::min<int64_t>((m + 3) / 4, (int64_t)ctx->num_cu * 16);
+      hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, (int64_t*)ctx->row_flp.p,
+                         &c->flops);
+    }
+    Phase ph(ctx, AMS_PH_CLASSIFY, sync);
+    const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
+    hipLaunchKernelGGL(k_classify_sym, dim3(gc), dim3(256), 0, ctx->stream, m,
+                       (const int64_t*)ctx->row_flp.p, pr, sp, (uint8_t*)ctx->sbin.p, rp_c, c->cnt_sym);
+    hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->sbin.p,
+                       kSymBins, (const int32_t*)c->cnt_sym, c->cur_sym, (int32_t*)ctx->lsym.p);
+    HIPCHK(hipGetLastError());
+  }
+  ctx->plan_counts = false;
+  if (sync) {
+    HIPCHK(hipMemcpyAsync(ctx->h_ctrl, c, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(ctx->cnt_sym, ctx->h_ctrl->cnt_sym, sizeof(ctx->cnt_sym));
+    ctx->plan_counts = true;
+    ctx->stats.flops = (int64_t)ctx->h_ctrl->flops;
+  }
+  const int64_t H = ctx->plan_counts ? ctx->cnt_sym[SB_BITMAP] : m;
+  ctx->plan_H = H;
+  CHK(grow(ctx, ctx->coff, sizeof(int32_t) * std::max<int64_t>(H, 1) * (R + 1)));
+
+  if (m > 0) {
+    int64_t* rownz = rp_c;
+    Phase ph(ctx, AMS_PH_SYMB_HASH, sync);
+    launch_symb_hash<64>(ctx, A, B, rownz, 0);
+    launch_symb_hash<128>(ctx, A, B, rownz, 1);
+    launch_symb_hash<256>(ctx, A, B, rownz, 2);
+    launch_symb_hash<512>(ctx, A, B, rownz, 3);
+    launch_symb_hash<1024>(ctx, A, B, rownz, 4);
+    launch_symb_hash<2048>(ctx, A, B, rownz, 5);
+    launch_symb_hash<4096>(ctx, A, B, rownz, 6);
+    launch_symb_hash<8192>(ctx, A, B, rownz, 7);
+    HIPCHK(hipGetLastError());
+  }
+  if (m > 0) {
+    int64_t* rownz = rp_c;
+    Phase ph(ctx, AMS_PH_SYMB_BITMAP, sync);
+    if (!ctx->plan_counts || H > 0) {
+      const size_t dyn = (size_t)(Wb / 32) * 4 + (size_t)(Wb / W + 1) * 4;
+      if (dyn + sizeof(ExpandLds<kSymBitmapT>) > ctx->lds_max)
+        return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
+      HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<kSymBitmapT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+      int occ;
+      occupancy(k_symb_bitmap<kSymBitmapT>, kSymBitmapT, dyn, &occ);
+      const int g = grid_for(ctx, occ, H, ctx->plan_counts);
+      hipLaunchKernelGGL(k_symb_bitmap<kSymBitmapT>, dim3(g), dim3(kSymBitmapT), dyn, ctx->stream, A, B, pr,
+                         (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
+                         (int32_t*)ctx->coff.p, R, W, Wb);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  {
+    Phase ph(ctx, AMS_PH_SCAN, sync);
+    CHK(fill_i64(ctx, rp_c + m, 1, 0));
+    CHK(scan_i64(ctx, rp_c, rp_c, m + 1));
+  }
+
+  NumParams np{ctx->prm.num_hash_max, ctx->prm.esc_max, ctx->prm.acc_thres, ctx->prm.dense_density,
+               ctx->prm.mode};
+  if (m > 0) {
+    Phase ph(ctx, AMS_PH_CLASSIFY, sync);
+    const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
+    hipLaunchKernelGGL(k_classify_num, dim3(gc), dim3(256), 0, ctx->stream, m,
+                       (const int64_t*)ctx->row_flp.p, (const int64_t*)rp_c, (const uint8_t*)ctx->sbin.p, pr,
+                       np, (uint8_t*)ctx->nbin.p, c->cnt_num);
+    hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->nbin.p,
+                       kNumBins, (const int32_t*)c->cnt_num, c->cur_num, (int32_t*)ctx->lnum.p);
+    if (sync)
+      hipLaunchKernelGGL(k_route_sums, dim3(gc), dim3(256), 0, ctx->stream, m, (const int64_t*)ctx->row_flp.p,
+                         (const int64_t*)rp_c, A.rp, (const uint8_t*)ctx->sbin.p, (const uint8_t*)ctx->nbin.p,
+                         c->sums);
+    HIPCHK(hipGetLastError());
+  }
+  if (sync) {
+    HIPCHK(hipMemcpyAsync(ctx->h_ctrl, c, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_pin, rp_c + m, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(ctx->cnt_num, ctx->h_ctrl->cnt_num, sizeof(ctx->cnt_num));
+    if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "symbolic kernels raised error flags 0x%x", ctx->h_ctrl->err);
+    if (nnz) *nnz = ctx->h_pin[0];
+    ams_stats_t& s = ctx->stats;
+    s.rows_total = m;
+    s.rows_sym_bitmap = ctx->cnt_sym[SB_BITMAP];
+    s.rows_sym_hash = 0;
+    for (int i = 0; i < kSymHashBins; ++i) s.rows_sym_hash += ctx->cnt_sym[i];
+    s.rows_empty = m - s.rows_sym_hash - s.rows_sym_bitmap;
+    s.rows_num_hash = s.rows_num_esc = 0;
+    for (int i = 0; i < kNumHashBins; ++i) s.rows_num_hash += ctx->cnt_num[i];
+    for (int i = 0; i < kNumEscBins; ++i) s.rows_num_esc += ctx->cnt_num[kNumHashBins + i];
+    s.rows_num_dense = ctx->cnt_num[NB_DENSE];
+    s.dense_chunks = R;
+    s.nnz_c = ctx->h_pin[0];
+    for (int k = 0; k < 3; ++k) {
+      s.flops_route[k] = (int64_t)ctx->h_ctrl->sums[k];
+      s.nnzc_route[k] = (int64_t)ctx->h_ctrl->sums[3 + k];
+      s.nnza_route[k] = (int64_t)ctx->h_ctrl->sums[6 + k];
+    }
+    for (int k = 0; k < 2; ++k) {
+      s.flops_sym[k] = (int64_t)ctx->h_ctrl->sums[9 + k];
+      s.nnza_sym[k] = (int64_t)ctx->h_ctrl->sums[11 + k];
+    }
+  }
+  ctx->plan_valid = true;
+  return AMS_OK;
+}
+
+// ------------------------------------------------------------- numeric
+
+template <int S, typename V>
+static void launch_numc_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
+                             V* val_c, int64_t cap, int bin) {
+  constexpr int T = hash_threads<S>();
+  const int64_t cnt = ctx->plan_counts ? ctx->cnt_num[bin] : A.m;
+  if (ctx->plan_counts && cnt == 0) return;
+  int occ;
+  occupancy(k_numc_hash<S, T, V>, T, 0, &occ);
+  const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+  Ctrl* c = dctrl(ctx);
+  hipLaunchKernelGGL((k_numc_hash<S, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                     (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin, rp_c, col_c, val_c, cap,
+                     &c->err);
+}
+
+template <int E, typename V>
+static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
+                            V* val_c, int64_t cap, int bin) {
+  constexpr int T = hash_threads<E>();
+  const int64_t cnt = ctx->plan_counts ? ctx->cnt_num[bin] : A.m;
+  if (ctx->plan_counts && cnt == 0) return;
+  int occ;
+  occupancy(k_numc_esc<E, T, V>, T, 0, &occ);
+  const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+  Ctrl* c = dctrl(ctx);
+  hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                     (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
+                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, &c->err);
+}
+
+// Column-blocked copy of B (chunk-major CSR of width-W column blocks).
+template <typename V>
+static int build_bblk(ams_ctx_t* ctx, const Csr& B, const Probs& pr, int W, int R, bool perm) {
+  const int64_t ld = B.m + 1;
+  CHK(grow(ctx, ctx->bptr, sizeof(int64_t) * (size_t)R * ld));
+  CHK(grow(ctx, ctx->bcol, sizeof(uint16_t) * std::max<int64_t>(B.nnz, 1)));
+  CHK(grow(ctx, ctx->bval, sizeof(V) * std::max<int64_t>(B.nnz, 1)));
+  if (perm) CHK(grow(ctx, ctx->bperm, sizeof(int64_t) * std::max<int64_t>(B.nnz, 1)));
+  int64_t* bptr = (int64_t*)ctx->bptr.p;
+  CHK(fill_i64(ctx, bptr, (int64_t)R * ld, 0));
+  if (B.m > 0) {
+    const int g = (int)std::min<int64_t>(B.m, (int64_t)ctx->num_cu * 32);
+    const size_t dyn = sizeof(int32_t) * R;
+    hipLaunchKernelGGL(k_bblk_count, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R, bptr);
+    CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
+    hipLaunchKernelGGL(k_bblk_scatter<V>, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R,
+                       (const int64_t*)bptr, (uint16_t*)ctx->bcol.p, (V*)ctx->bval.p,
+                       perm ? (int64_t*)ctx->bperm.p : (int64_t*)nullptr);
+    HIPCHK(hipGetLastError());
+  } else {
+    CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
+  }
+  return AMS_OK;
+}
+
+template <typename V>
+static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, const int64_t* rp_c,
+                         int32_t* col_c, V* val_c, int64_t cap) {
+  const Csr A = to_csr(a), B = to_csr(b);
+  const Probs& pr = ctx->plan_pr;
+  Ctrl* c = dctrl(ctx);
+  const int W = ctx->plan_W, R = ctx->plan_R;
+  const bool dense = !ctx->plan_counts || ctx->cnt_num[NB_DENSE] > 0;
+  const bool on = ctx->plan_counts;
+  if (dense && A.m > 0) {
+    {
+      Phase ph(ctx, AMS_PH_BBLK, on);
+      CHK(build_bblk<V>(ctx, B, pr, W, R, false));
+    }
+    Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
+    const size_t dyn = dense_lds(W, sizeof(V));
+    if (dyn + sizeof(V) * kDenseT + 8 * (kDenseT / 64) + 16 > ctx->lds_max)
+      return fail(ctx, AMS_ERR_ARG, "dense_width %d needs %zu B of LDS", W, dyn);
+    HIPCHK(hipFuncSetAttribute((const void*)k_numc_dense<kDenseT, V>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
+    int occ;
+    occupancy(k_numc_dense<kDenseT, V>, kDenseT, dyn, &occ);
+    const int64_t tasks = ctx->plan_counts ? (int64_t)ctx->cnt_sym[SB_BITMAP] * R : A.m;
+    const int g = grid_for(ctx, occ, tasks, ctx->plan_counts);
+    hipLaunchKernelGGL((k_numc_dense<kDenseT, V>), dim3(g), dim3(kDenseT), dyn, ctx->stream, A, B.m,
+                       (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
+                       (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP,
+                       (const uint8_t*)ctx->nbin.p, (const int32_t*)ctx->coff.p, R, W, rp_c, col_c, val_c, cap,
+                       pr, &c->ticket, &c->err);
+    HIPCHK(hipGetLastError());
+  }
+  if (A.m > 0) {
+    Phase ph(ctx, AMS_PH_NUMC_HASH, on);
+    launch_numc_hash<64, V>(ctx, A, B, rp_c, col_c, val_c, cap, 0);
+    launch_numc_hash<128, V>(ctx, A, B, rp_c, col_c, val_c, cap, 1);
+    launch_numc_hash<256, V>(ctx, A, B, rp_c, col_c, val_c, cap, 2);
+    launch_numc_hash<512, V>(ctx, A, B, rp_c, col_c, val_c, cap, 3);
+    launch_numc_hash<1024, V>(ctx, A, B, rp_c, col_c, val_c, cap, 4);
+    launch_numc_hash<2048, V>(ctx, A, B, rp_c, col_c, val_c, cap, 5);
+    launch_numc_hash<4096, V>(ctx, A, B, rp_c, col_c, val_c, cap, 6);
+    launch_numc_hash<8192, V>(ctx, A, B, rp_c, col_c, val_c, cap, 7);
+  }
+  if (A.m > 0) {
+    Phase ph(ctx, AMS_PH_NUMC_ESC, on);
+    launch_numc_esc<64, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 0);
+    launch_numc_esc<128, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 1);
+    launch_numc_esc<256, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 2);
+    launch_numc_esc<512, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 3);
+    launch_numc_esc<1024, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 4);
+    launch_numc_esc<2048, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 5);
+    launch_numc_esc<4096, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 6);
+    HIPCHK(hipGetLastError());
+  }
+  return AMS_OK;
+}
+
+static int run_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, const int64_t* rp_c,
+                       int32_t* col_c, void* val_c, int64_t cap) {
+  if (dtype == AMS_F64) return run_numeric_t<double>(ctx, a, b, rp_c, col_c, (double*)val_c, cap);
+  return run_numeric_t<float>(ctx, a, b, rp_c, col_c, (float*)val_c, cap);
+}
+
+static int check_pair(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype) {
+  if (!ctx) return AMS_ERR_ARG;
+  CHK(check_csr(ctx, a, "A"));
+  CHK(check_csr(ctx, b, "B"));
+  if (a->num_col != b->num_row)
+    return fail(ctx, AMS_ERR_ARG, "A is %lld x %lld but B has %lld rows", (long long)a->num_row,
+                (long long)a->num_col, (long long)b->num_row);
+  if (dtype != AMS_F64 && dtype != AMS_F32) return fail(ctx, AMS_ERR_ARG, "unknown dtype %d", dtype);
+  return AMS_OK;
+}
+
+static Probs single_probs(const ams_dcsr_t* b) {
+  Probs pr{};
+  pr.n = 1;
+  pr.width0 = b->num_col;
+  pr.max_width = b->num_col;
+  return pr;
+}
+
+// ----------------------------------------------------------------- C ABI
+
+extern "C" {
+
+void ams_default_params(ams_params_t* p) {
+  p->sym_hash_max = 2730;
+  p->num_hash_max = 2730;
+  p->esc_max = 4096;
+  p->acc_thres = 2.0;
+  p->dense_width = 7424;
+  p->bitmap_bits = 1 << 19;
+  p->bitmap_ratio = 1.0;
+  p->dense_density = 0.05;
+  p->mode = AMS_MODE_HYBRID;
+  p->wgs_per_cu = 0;
+}
+
+int ams_ctx_create(ams_ctx_t** out, void* stream) {
+  if (!out) return AMS_ERR_ARG;
+  ams_ctx_t* ctx = new ams_ctx_t();
+  ctx->stream = (hipStream_t)stream;
+  ams_default_params(&ctx->prm);
+  if (hipGetDevice(&ctx->device) != hipSuccess) {
+    (void)hipGetLastError();
+    delete ctx;
+    return AMS_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) {
+    ctx->num_cu = prop.multiProcessorCount;
+    ctx->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+  }
+  if (hipHostMalloc((void**)&ctx->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_pin, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    delete ctx;
+    return AMS_ERR_ALLOC;
+  }
+  *out = ctx;
+  return AMS_OK;
+}
+
+int ams_ctx_destroy(ams_ctx_t* ctx) {
+  if (!ctx) return AMS_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  Buf* bufs[] = {&ctx->row_flp, &ctx->sbin, &ctx->nbin, &ctx->lsym, &ctx->lnum, &ctx->ctrl, &ctx->coff,
+                 &ctx->bptr, &ctx->bcol, &ctx->bval, &ctx->bperm, &ctx->tsum, &ctx->pool_rp, &ctx->pool_col,
+                 &ctx->pool_val, &ctx->probs, &ctx->scratch_a};
+  for (Buf* b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (auto& r : ctx->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : ctx->ev_free) (void)hipEventDestroy(e);
+  if (ctx->h_ctrl) (void)hipHostFree(ctx->h_ctrl);
+  if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+  delete ctx;
+  return AMS_OK;
+}
+
+int ams_set_stream(ams_ctx_t* ctx, void* stream) {
+  if (!ctx) return AMS_ERR_ARG;
+  ctx->stream = (hipStream_t)stream;
+  return AMS_OK;
+}
+
+const char* ams_last_error(const ams_ctx_t* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
+  if (!ctx || !p) return AMS_ERR_ARG;
+  if (p->sym_hash_max < 1 || p->sym_hash_max > (kMaxHashSlots * 2) / 3)
+    return fail(ctx, AMS_ERR_ARG, "sym_hash_max must be in [1, %d]", (kMaxHashSlots * 2) / 3);
+  if (p->num_hash_max < p->sym_hash_max || p->num_hash_max > (kMaxHashSlots * 2) / 3)
+    return fail(ctx, AMS_ERR_ARG, "num_hash_max must be in [sym_hash_max, %d]", (kMaxHashSlots * 2) / 3);
+  if (p->esc_max < 0 || p->esc_max > kMaxEsc) return fail(ctx, AMS_ERR_ARG, "esc_max must be in [0, %d]", kMaxEsc);
+  if (p->dense_width < 64 || p->dense_width > 65536 || p->dense_width % 64)
+    return fail(ctx, AMS_ERR_ARG, "dense_width must be a multiple of 64 in [64, 65536]");
+  if (p->bitmap_bits < p->dense_width) return fail(ctx, AMS_ERR_ARG, "bitmap_bits must be >= dense_width");
+  if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
+  ctx->prm = *p;
+  return AMS_OK;
+}
+
+int ams_get_params(const ams_ctx_t* ctx, ams_params_t* p) {
+  if (!ctx || !p) return AMS_ERR_ARG;
+  *p = ctx->prm;
+  return AMS_OK;
+}
+
+int ams_get_stats(const ams_ctx_t* ctx, ams_stats_t* s) {
+  if (!ctx || !s) return AMS_ERR_ARG;
+  *s = ctx->stats;
+  return AMS_OK;
+}
+
+int ams_row_flops(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int64_t* row_flp, int64_t* total) {
+  CHK(check_pair(ctx, a, b, AMS_F64));
+  CHK(grow(ctx, ctx->ctrl, sizeof(Ctrl)));
+  Ctrl* c = dctrl(ctx);
+  CHK(fill_i64(ctx, (int64_t*)&c->flops, 1, 0));
+  const Csr A = to_csr(a), B = to_csr(b);
+  if (A.m > 0) {
+    const int g = (int)std::min<int64_t>((A.m + 3) / 4, (int64_t)ctx->num_cu * 16);
+    hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, row_flp, &c->flops);
+    HIPCHK(hipGetLastError());
+  }
+  if (total) {
+    HIPCHK(hipMemcpyAsync(ctx->h_pin, &c->flops, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *total = ctx->h_pin[0];
+  }
+  return AMS_OK;
+}
+
+int ams_scan_i64(ams_ctx_t* ctx, const int64_t* in, int64_t* out, int64_t n) {
+  if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, AMS_ERR_ARG, "bad scan arguments");
+  // n outputs from in[0..n-1): the last input is not read.
+  if (n == 0) return AMS_OK;
+  if (in != out) {
+    HIPCHK(hipMemcpyAsync(out, in, sizeof(int64_t) * (n - 1), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  CHK(fill_i64(ctx, out + n - 1, 1, 0));
+  return scan_i64(ctx, out, out, n);
+}
+
+int ams_spgemm_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int64_t* rp_c,
+                        int64_t* nnz) {
+  CHK(check_pair(ctx, a, b, dtype));
+  if (!rp_c) return fail(ctx, AMS_ERR_ARG, "row_ptr_c is NULL");
+  return run_symbolic(ctx, a, b, dtype, single_probs(b), rp_c, true, nnz);
+}
+
+int ams_spgemm_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, const int64_t* rp_c,
+                       int32_t* col_c, void* val_c) {
+  CHK(check_pair(ctx, a, b, dtype));
+  if (!ctx->plan_valid || ctx->plan_m != a->num_row || ctx->plan_mB != b->num_row || ctx->plan_dtype != dtype)
+    return fail(ctx, AMS_ERR_STATE, "numeric needs the symbolic phase of the same A and B first");
+  if (!rp_c || (ctx->stats.nnz_c > 0 && (!col_c || !val_c)))
+    return fail(ctx, AMS_ERR_ARG, "output buffers are NULL");
+  return run_numeric(ctx, a, b, dtype, rp_c, col_c, val_c, INT64_MAX);
+}
+
+int ams_spgemm(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, ams_dcsr_t* c_out) {
+  CHK(check_pair(ctx, a, b, dtype));
+  if (!c_out) return fail(ctx, AMS_ERR_ARG, "c_out is NULL");
+  CHK(grow(ctx, ctx->pool_rp, sizeof(int64_t) * (a->num_row + 1)));
+  int64_t* rp = (int64_t*)ctx->pool_rp.p;
+  int64_t nnz = 0;
+  CHK(run_symbolic(ctx, a, b, dtype, single_probs(b), rp, true, &nnz));
+  const size_t vb = dtype == AMS_F64 ? 8 : 4;
+  CHK(grow(ctx, ctx->pool_col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+  CHK(grow(ctx, ctx->pool_val, vb * std::max<int64_t>(nnz, 1)));
+  CHK(run_numeric(ctx, a, b, dtype, rp, (int32_t*)ctx->pool_col.p, ctx->pool_val.p, INT64_MAX));
+  HIPCHK(hipMemcpyAsync(&ctx->h_ctrl->err, &dctrl(ctx)->err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "numeric kernels raised error flags 0x%x", ctx->h_ctrl->err);
+  c_out->num_row = a->num_row;
+  c_out->num_col = b->num_col;
+  c_out->num_nnz = nnz;
+  c_out->row_ptr = rp;
+  c_out->col = (const int32_t*)ctx->pool_col.p;
+  c_out->val = ctx->pool_val.p;
+  return AMS_OK;
+}
+
+int ams_verify_result(ams_ctx_t* ctx, const ams_dcsr_t* c, int dtype, int64_t* nnz, int64_t* sum_col,
+                      double* sum_val, int64_t* unsorted) {
+  if (!ctx) return AMS_ERR_ARG;
+  CHK(check_csr(ctx, c, "C"));
+  CHK(grow(ctx, ctx->ctrl, sizeof(Ctrl)));
+  Ctrl* d = dctrl(ctx);
+  unsigned long long* out = &d->sums[13];  // [13] col sum, [14] unsorted rows, [15] value sum
+  CHK(fill_i64(ctx, (int64_t*)out, 3, 0));
+  const Csr C = to_csr(c);
+  if (C.m > 0) {
+    const int g = (int)std::min<int64_t>((C.m + 3) / 4, (int64_t)ctx->num_cu * 16);
+    if (dtype == AMS_F64)
+      hipLaunchKernelGGL(k_verify<double>, dim3(g), dim3(256), 0, ctx->stream, C, out, (double*)(out + 2));
+    else
+      hipLaunchKernelGGL(k_verify<float>, dim3(g), dim3(256), 0, ctx->stream, C, out, (double*)(out + 2));
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(ctx->h_pin, out, 24, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->h_pin + 3, c->row_ptr + c->num_row, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (sum_col) *sum_col = ctx->h_pin[0];
+  if (unsorted) *unsorted = ctx->h_pin[1];
+  if (sum_val) std::memcpy(sum_val, &ctx->h_pin[2], 8);
+  if (nnz) *nnz = ctx->h_pin[3];
+  return AMS_OK;
+}
+
+int ams_enable_timing(ams_ctx_t* ctx, int on) {
+  if (!ctx) return AMS_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->recs) {
+    ctx->ev_free.push_back(r.a);
+    ctx->ev_free.push_back(r.b);
+  }
+  ctx->recs.clear();
+  for (int i = 0; i < AMS_PH_COUNT; ++i) {
+    ctx->acc_ms[i] = 0;
+    ctx->acc_n[i] = 0;
+  }
+  ctx->timing = on != 0;
+  return AMS_OK;
+}
+
+int ams_get_timings(ams_ctx_t* ctx, double* ms, int64_t* launches) {
+  if (!ctx) return AMS_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->recs) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+    ctx->acc_ms[r.ph] += t;
+    ctx->acc_n[r.ph] += 1;
+    ctx->ev_free.push_back(r.a);
+    ctx->ev_free.push_back(r.b);
+  }
+  ctx->recs.clear();
+  for (int i = 0; i < AMS_PH_COUNT; ++i) {
+    if (ms) ms[i] = ctx->acc_ms[i];
+    if (launches) launches[i] = ctx->acc_n[i];
+  }
+  return AMS_OK;
+}
+
+int ams_dev_alloc(void** ptr, int64_t bytes) {
+  if (!ptr || bytes < 0) return AMS_ERR_ARG;
+  if (hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 8) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return AMS_ERR_ALLOC;
+  }
+  return AMS_OK;
+}
+
+int ams_dev_free(void* ptr) {
+  if (ptr && hipFree(ptr) != hipSuccess) {
+    (void)hipGetLastError();
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+int ams_memcpy_h2d(ams_ctx_t* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx) return AMS_ERR_ARG;
+  if (bytes <= 0) return AMS_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return AMS_OK;
+}
+
+int ams_memcpy_d2h(ams_ctx_t* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx) return AMS_ERR_ARG;
+  if (bytes <= 0) return AMS_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return AMS_OK;
+}
+
+int ams_sync(ams_ctx_t* ctx) {
+  if (!ctx) return AMS_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return AMS_OK;
+}
+
+int ams_i64_to_i32(ams_ctx_t* ctx, const int64_t* in, int32_t* out, int64_t n) {
+  if (!ctx) return AMS_ERR_ARG;
+  if (n <= 0) return AMS_OK;
+  const int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_i64_to_i32, dim3(g), dim3(256), 0, ctx->stream, in, out, n);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+int ams_i32_to_i64(ams_ctx_t* ctx, const int32_t* in, int64_t* out, int64_t n) {
+  if (!ctx) return AMS_ERR_ARG;
+  if (n <= 0) return AMS_OK;
+  const int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_i32_to_i64, dim3(g), dim3(256), 0, ctx->stream, in, out, n);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+int ams_batched_plan(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int32_t nprob,
+                     const int64_t* arow, const int64_t* brow, const int64_t* ccol, int64_t* cap) {
+  CHK(check_pair(ctx, a, b, dtype));
+  ctx->batch_ready = false;
+  if (nprob < 1 || !arow || !brow || !ccol || !cap) return fail(ctx, AMS_ERR_ARG, "bad problem table");
+  if (arow[0] != 0 || brow[0] != 0 || ccol[0] != 0 || arow[nprob] != a->num_row || brow[nprob] != a->num_col ||
+      ccol[nprob] != b->num_col)
+    return fail(ctx, AMS_ERR_ARG, "problem offsets do not span the stacked matrices");
+  int64_t maxw = 0;
+  for (int i = 0; i < nprob; ++i) {
+    if (arow[i + 1] < arow[i] || brow[i + 1] < brow[i] || ccol[i + 1] < ccol[i])
+      return fail(ctx, AMS_ERR_ARG, "problem offsets must be non-decreasing");
+    maxw = std::max(maxw, ccol[i + 1] - ccol[i]);
+  }
+  Probs pr{};
+  pr.n = nprob;
+  pr.width0 = b->num_col;
+  pr.max_width = nprob == 1 ? b->num_col : maxw;
+  if (nprob > 1) {
+    CHK(grow(ctx, ctx->probs, sizeof(int64_t) * 3 * (nprob + 1)));
+    int64_t* d = (int64_t*)ctx->probs.p;
+    std::vector<int64_t> h(3 * (size_t)(nprob + 1));
+    std::memcpy(h.data(), arow, sizeof(int64_t) * (nprob + 1));
+    std::memcpy(h.data() + (nprob + 1), brow, sizeof(int64_t) * (nprob + 1));
+    std::memcpy(h.data() + 2 * (nprob + 1), ccol, sizeof(int64_t) * (nprob + 1));
+    HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(int64_t) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    pr.arow = d;
+    pr.brow = d + (nprob + 1);
+    pr.ccol = d + 2 * (nprob + 1);
+  }
+  const int64_t m = a->num_row;
+  CHK(grow(ctx, ctx->pool_rp, sizeof(int64_t) * (m + 1)));
+  int64_t nnz = 0;
+  CHK(run_symbolic(ctx, a, b, dtype, pr, (int64_t*)ctx->pool_rp.p, true, &nnz));
+  // size every buffer for the unsynchronised run: all rows may be heavy
+  int W, R, Wb;
+  dense_geometry(ctx, dtype, pr.max_width, W, R, Wb);
+  const int64_t ld = b->num_row + 1;
+  CHK(grow(ctx, ctx->coff, sizeof(int32_t) * std::max<int64_t>(m, 1) * (R + 1)));
+  CHK(grow(ctx, ctx->bptr, sizeof(int64_t) * (size_t)R * ld));
+  CHK(grow(ctx, ctx->bcol, sizeof(uint16_t) * std::max<int64_t>(b->num_nnz, 1)));
+  CHK(grow(ctx, ctx->bval, (dtype == AMS_F64 ? 8 : 4) * std::max<int64_t>(b->num_nnz, 1)));
+  const int64_t nmax = std::max<int64_t>(m + 1, (int64_t)R * ld);
+  CHK(grow(ctx, ctx->tsum, sizeof(int64_t) * ((nmax + kScanTile - 1) / kScanTile)));
+  ctx->batch_pr = pr;
+  ctx->batch_m = m;
+  ctx->batch_mB = b->num_row;
+  ctx->batch_nnzA = a->num_nnz;
+  ctx->batch_nnzB = b->num_nnz;
+  ctx->batch_dtype = dtype;
+  ctx->batch_ready = true;
+  *cap = nnz;
+  return AMS_OK;
+}
+
+int ams_batched_run(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int64_t* c_rp,
+                    int32_t* c_col, void* c_val, int64_t cap, int64_t* nnz_dev) {
+  CHK(check_pair(ctx, a, b, dtype));
+  if (!ctx->batch_ready || ctx->batch_m != a->num_row || ctx->batch_mB != b->num_row ||
+      ctx->batch_nnzA != a->num_nnz || ctx->batch_nnzB != b->num_nnz || ctx->batch_dtype != dtype)
+    return fail(ctx, AMS_ERR_STATE, "ams_batched_run needs ams_batched_plan of the same inputs");
+  if (!c_rp || !nnz_dev || (cap > 0 && (!c_col || !c_val))) return fail(ctx, AMS_ERR_ARG, "NULL output buffer");
+  CHK(run_symbolic(ctx, a, b, dtype, ctx->batch_pr, c_rp, false, nullptr));
+  CHK(run_numeric(ctx, a, b, dtype, c_rp, c_col, c_val, cap));
+  hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, ctx->stream, (const int64_t*)(c_rp + a->num_row), nnz_dev);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+int ams_batched_capture(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int64_t* c_rp,
+                        int32_t* c_col, void* c_val, int64_t cap, int64_t* nnz_dev, void** exec) {
+  if (!ctx || !exec) return AMS_ERR_ARG;
+  if (!ctx->stream) return fail(ctx, AMS_ERR_ARG, "graph capture needs a non-null stream");
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  const int r = ams_batched_run(ctx, a, b, dtype, c_rp, c_col, c_val, cap, nnz_dev);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (r != AMS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(ctx, AMS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  hipGraphExec_t ge = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) return fail(ctx, AMS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+  *exec = (void*)ge;
+  return AMS_OK;
+}
+
+int ams_graph_launch(ams_ctx_t* ctx, void* exec) {
+  if (!ctx || !exec) return AMS_ERR_ARG;
+  HIPCHK(hipGraphLaunch((hipGraphExec_t)exec, ctx->stream));
+  return AMS_OK;
+}
+
+int ams_graph_destroy(void* exec) {
+  if (exec && hipGraphExecDestroy((hipGraphExec_t)exec) != hipSuccess) {
+    (void)hipGetLastError();
+    return AMS_ERR_HIP;
+  }
+  return AMS_OK;
+}
+
+// Coordinate search over the routing thresholds and LDS geometry, timing the
+// whole one-shot product with events on the context stream.
+int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int reps,
+                 ams_params_t* best_out, double* best_ms_out) {
+  CHK(check_pair(ctx, a, b, dtype));
+  if (reps < 1) reps = 1;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto timed = [&](const ams_params_t& p, double* ms) -> int {
+    CHK(ams_set_params(ctx, &p));
+    ams_dcsr_t c;
+    CHK(ams_spgemm(ctx, a, b, dtype, &c));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < reps; ++i) CHK(ams_spgemm(ctx, a, b, dtype, &c));
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = (double)t / reps;
+    return AMS_OK;
+  };
+  ams_params_t best = ctx->prm;
+  double best_ms = 0;
+  int r = timed(best, &best_ms);
+  if (r != AMS_OK) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+  }
+  const int hash_pairs[][2] = {{682, 682}, {1365, 1365}, {2730, 2730}, {1365, 2730}, {2730, 5461}, {5461, 5461}};
+  const int widths[] = {3968, 7424, 12800, 15360, 24576};
+  const double thres[] = {0.0, 1.25, 2.0, 4.0};
+  const double ratios[] = {0.25, 1.0, 4.0};
+  const double dens[] = {0.02, 0.05, 0.2, 1.0};
+  auto consider = [&](ams_params_t p) {
+    double ms;
+    if (timed(p, &ms) == AMS_OK && ms < best_ms) {
+      best_ms = ms;
+      best = p;
+    }
+  };
+  for (auto& hp : hash_pairs) { ams_params_t p = best; p.sym_hash_max = hp[0]; p.num_hash_max = hp[1]; consider(p); }
+  for (int w : widths) { ams_params_t p = best; p.dense_width = w; if (p.bitmap_bits < w) p.bitmap_bits = w; consider(p); }
+  for (double t : thres) { ams_params_t p = best; p.acc_thres = t; consider(p); }
+  for (double q : ratios) { ams_params_t p = best; p.bitmap_ratio = q; consider(p); }
+  for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->err.clear();
+  CHK(ams_set_params(ctx, &best));
+  if (best_out) *best_out = best;
+  if (best_ms_out) *best_ms_out = best_ms;
+  return AMS_OK;
+}
+
+int ams_spgemm_masked_bwd(ams_ctx_t* ctx, const ams_dcsr_t*, const ams_dcsr_t*, const ams_dcsr_t*, int, void*,
+                          void*) {
+  return fail(ctx, AMS_ERR_STATE, "masked backward is not built yet");
+}
+
+}  // extern "C"
