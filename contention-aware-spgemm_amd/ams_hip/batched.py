@@ -1,0 +1,121 @@
+"""Batched form: many independent products C_i = A_i * B_i as one
+block-diagonal product, planned once and replayed as one captured hipGraph
+(ams_batched_plan / ams_batched_capture / ams_graph_launch of the C ABI).
+The reference has no batched form.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .csr import Csr
+from .device import Context, DeviceCsr
+
+
+def stack(problems: List[Tuple[Csr, Csr]]):
+    """Block-diagonal stacking.  Returns (A_stack, B_stack, a_row_off,
+    b_row_off, c_col_off) with A_i's columns shifted to B_i's rows and B_i's
+    columns shifted to C_i's output columns."""
+    a_row = [0]
+    b_row = [0]
+    c_col = [0]
+    for a, b in problems:
+        if a.num_col != b.num_row:
+            raise ValueError("A_i columns must equal B_i rows")
+        a_row.append(a_row[-1] + a.num_row)
+        b_row.append(b_row[-1] + b.num_row)
+        c_col.append(c_col[-1] + b.num_col)
+
+    def cat(mats, col_shift):
+        rps, cols, vals = [np.zeros(1, np.int64)], [], []
+        base = 0
+        for (m, sh) in zip(mats, col_shift):
+            rps.append(np.asarray(m.row_ptrs[1:], np.int64) + base)
+            base += m.num_nnz
+            cols.append(np.asarray(m.cols, np.int64) + sh)
+            vals.append(m.rats)
+        return np.concatenate(rps), np.concatenate(cols).astype(np.int32), np.concatenate(vals)
+
+    rp, cols, vals = cat([p[0] for p in problems], b_row[:-1])
+    A = Csr(a_row[-1], b_row[-1], rp, cols, vals)
+    rp, cols, vals = cat([p[1] for p in problems], c_col[:-1])
+    B = Csr(b_row[-1], c_col[-1], rp, cols, vals)
+    return A, B, np.array(a_row, np.int64), np.array(b_row, np.int64), np.array(c_col, np.int64)
+
+
+class BatchedProduct:
+    """Plans a batch once; ``run()`` replays one captured graph."""
+
+    def __init__(self, problems: List[Tuple[Csr, Csr]], ams: Context | None = None, capture: bool = True):
+        self.problems = problems
+        self.stream = torch.cuda.Stream() if ams is None else ams.stream
+        self.ams = ams or Context(stream=self.stream)
+        A, B, self.a_off, self.b_off, self.c_off = stack(problems)
+        self.A = DeviceCsr.from_host(A)
+        self.B = DeviceCsr.from_host(B)
+        torch.cuda.synchronize()  # uploads ran on the current stream, the plan runs on self.stream
+        lib = L.lib()
+        sa, sb = self.A.c_struct(), self.B.c_struct()
+        cap = C.c_int64()
+        ptr = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self.ams._chk(lib.ams_batched_plan(self.ams.handle, C.byref(sa), C.byref(sb), self.A.dtype_code,
+                                           len(problems), ptr(self.a_off), ptr(self.b_off), ptr(self.c_off),
+                                           C.byref(cap)), "ams_batched_plan")
+        self.cap = cap.value
+        dev = self.A.val.device
+        self.c_rp = torch.empty(A.num_row + 1, dtype=torch.int64, device=dev)
+        self.c_col = torch.empty(max(self.cap, 1), dtype=torch.int32, device=dev)
+        self.c_val = torch.empty(max(self.cap, 1), dtype=self.A.val.dtype, device=dev)
+        self.nnz = torch.zeros(1, dtype=torch.int64, device=dev)
+        self._sa, self._sb = sa, sb
+        self.graph = None
+        if capture:
+            g = C.c_void_p()
+            self.ams._chk(lib.ams_batched_capture(self.ams.handle, C.byref(sa), C.byref(sb), self.A.dtype_code,
+                                                  self.c_rp.data_ptr(), self.c_col.data_ptr(), self.c_val.data_ptr(),
+                                                  self.cap, self.nnz.data_ptr(), C.byref(g)),
+                          "ams_batched_capture")
+            self.graph = g
+
+    def run(self):
+        lib = L.lib()
+        if self.graph is not None:
+            self.ams._chk(lib.ams_graph_launch(self.ams.handle, self.graph), "ams_graph_launch")
+        else:
+            self.ams._chk(lib.ams_batched_run(self.ams.handle, C.byref(self._sa), C.byref(self._sb),
+                                              self.A.dtype_code, self.c_rp.data_ptr(), self.c_col.data_ptr(),
+                                              self.c_val.data_ptr(), self.cap, self.nnz.data_ptr()),
+                          "ams_batched_run")
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def results(self) -> List[Csr]:
+        """Per-problem products (host copies, columns rebased)."""
+        self.synchronize()
+        rp = self.c_rp.cpu().numpy()
+        nnz = int(rp[-1])
+        cols = self.c_col[:nnz].cpu().numpy().astype(np.int64)
+        vals = self.c_val[:nnz].cpu().numpy()
+        out = []
+        for i, (a, b) in enumerate(self.problems):
+            r0, r1 = self.a_off[i], self.a_off[i + 1]
+            s, e = rp[r0], rp[r1]
+            out.append(Csr(a.num_row, b.num_col, rp[r0:r1 + 1] - s, (cols[s:e] - self.c_off[i]).astype(np.int32),
+                           vals[s:e]))
+        return out
+
+    def close(self):
+        if self.graph is not None:
+            L.lib().ams_graph_destroy(self.graph)
+            self.graph = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

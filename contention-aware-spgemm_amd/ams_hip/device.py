@@ -143,6 +143,16 @@ class Context:
         self._chk(self._lib.ams_spgemm(self._p, C.byref(sa), C.byref(sb), a.dtype_code, C.byref(c)), "ams_spgemm")
         return c
 
+    def masked_bwd(self, a: DeviceCsr, b: DeviceCsr, g: DeviceCsr):
+        """dA = (G B^T) on pattern(A), dB = (A^T G) on pattern(B), for G on
+        pattern(A B) with ascending columns in every row."""
+        da = torch.empty(max(a.num_nnz, 1), dtype=a.val.dtype, device=a.val.device)
+        db = torch.empty(max(b.num_nnz, 1), dtype=b.val.dtype, device=b.val.device)
+        sa, sb, sg = a.c_struct(), b.c_struct(), g.c_struct()
+        self._chk(self._lib.ams_spgemm_masked_bwd(self._p, C.byref(sa), C.byref(sb), C.byref(sg), a.dtype_code,
+                                                  da.data_ptr(), db.data_ptr()), "ams_spgemm_masked_bwd")
+        return da[:a.num_nnz], db[:b.num_nnz]
+
     def verify(self, c, dtype_code: int = L.AMS_F64) -> dict:
         """VERIFYRESULT triple (nnz, sum of columns, sum of values) of a
         product, plus the count of rows whose columns are not ascending."""

@@ -1,0 +1,223 @@
+// ams_bwd_impl.hpp -- masked-SpGEMM backward of C = A * B (included once,
+// by ams_spgemm.hip).
+//
+// With the upstream gradient G on pattern(C):
+//   dA[i,k] = sum_j G[i,j] B[k,j]   over (i,k) in pattern(A)
+//   dB[k,j] = sum_i A[i,k] G[i,j]   over (k,j) in pattern(B)
+// Both come from one pass over the forward's products (i, k, j): every
+// product j of row i lies in pattern(C) row i, so G's row i is the
+// pre-seeded key set of the row's accumulator, looked up instead of
+// inserted.  dA[p] (p = A nonzero) is a sum over one B row: reduced in
+// registers and LDS and written once.  dB[q] (q = B nonzero) is shared
+// between rows: one global atomic per product at the known CSR slot q.
+// Cost: P lookups + P atomics, against P inserts of the forward.
+#pragma once
+
+#include "ams_kernels_impl.hpp"
+
+namespace ams {
+
+// Wave-segmented sum of v over consecutive lanes with equal key (keys are
+// non-decreasing across the wave); returns true on the first lane of each
+// run, which then holds the run's total in v.
+template <typename V>
+__device__ __forceinline__ bool wave_run_sum(int key, V& v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const V y = __shfl_down(v, d, kWave);
+    const int ky = __shfl_down(key, d, kWave);
+    if (lane + d < kWave && ky == key) v += y;
+  }
+  const int kp = __shfl_up(key, 1, kWave);
+  return lane == 0 || kp != key;
+}
+
+// One window-flattened pass over the products of A nonzeros [p0, p1), like
+// block_expand, but every lane runs every iteration (so the body may use
+// wave shuffles) and `flush(j, p)` runs per nonzero after each window.
+template <int T, class Seg, class Body, class Flush>
+__device__ __forceinline__ void block_expand_flush(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Body body,
+                                                   Flush flush) {
+  for (int64_t w0 = p0; w0 < p1; w0 += T) {
+    const int64_t p = w0 + threadIdx.x;
+    int64_t st = 0;
+    int32_t len = 0;
+    if (p < p1) seg(p, st, len);
+    int64_t total64;
+    const int64_t ex = block_excl_scan<T, int64_t>((int64_t)len, L.wtmp, total64);
+    L.off[threadIdx.x] = (int32_t)ex;
+    L.start[threadIdx.x] = st;
+    __syncthreads();
+    const int32_t total = (int32_t)total64;
+    const int nvalid = (int)((p1 - w0) < T ? (p1 - w0) : T);
+    for (int32_t t0 = 0; t0 < total; t0 += T) {
+      const int32_t t = t0 + threadIdx.x;
+      const bool valid = t < total;
+      int lo = 0;
+      if (valid) {
+        int hi = nvalid;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (L.off[mid] <= t) lo = mid; else hi = mid;
+        }
+      }
+      body(valid, valid ? lo : -1, w0 + lo, valid ? L.start[lo] + (int64_t)(t - L.off[lo]) : 0);
+    }
+    __syncthreads();
+    if (threadIdx.x < nvalid) flush(threadIdx.x, w0 + threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// Rows whose G row fits an LDS table: one row per block.
+template <int S, int T, typename V>
+__global__ __launch_bounds__(T) void k_bwd_hash(Csr A, Csr B, Csr G, const int32_t* list, const int32_t* counts,
+                                                int bin, V* dA, V* dB, int32_t* err) {
+  __shared__ int32_t keys[S];
+  __shared__ V gv[S];
+  __shared__ ExpandLds<T> L;
+  __shared__ V aval[T];
+  __shared__ V sda[T];
+  constexpr int shift = 32 - __builtin_ctz(S);
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const V* av = (const V*)A.val;
+  const V* bv = (const V*)B.val;
+  const V* gval = (const V*)G.val;
+  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+    const int32_t row = list[start + idx];
+    for (int s = threadIdx.x; s < S; s += T) keys[s] = kEmpty;
+    __syncthreads();
+    for (int64_t e = G.rp[row] + threadIdx.x; e < G.rp[row + 1]; e += T) {
+      const int32_t key = G.col[e];
+      const uint32_t mask = (0xffffffffu >> shift);
+      uint32_t h = hash_slot(key, shift);
+      uint32_t probe = 0;
+      while (atomicCAS(&keys[h], kEmpty, key) != kEmpty && probe <= mask) {
+        h = (h + 1) & mask;
+        ++probe;
+      }
+      if (probe > mask) atomicOr(err, 32);
+      else gv[h] = gval[e];
+    }
+    __syncthreads();
+    block_expand_flush<T>(
+        A.rp[row], A.rp[row + 1], L,
+        [&](int64_t p, int64_t& st, int32_t& len) {
+          const int32_t k = A.col[p];
+          st = B.rp[k];
+          len = (int32_t)(B.rp[k + 1] - st);
+          aval[threadIdx.x] = av[p];
+          sda[threadIdx.x] = V(0);
+        },
+        [&](bool valid, int j, int64_t, int64_t q) {
+          V contrib = V(0);
+          if (valid) {
+            const uint32_t slot = lds_find(keys, shift, B.col[q]);
+            const V g = slot < (uint32_t)S ? gv[slot] : V(0);
+            if (slot >= (uint32_t)S) atomicOr(err, 64);
+            unsafeAtomicAdd(&dB[q], aval[j] * g);
+            contrib = g * bv[q];
+          }
+          if (wave_run_sum<V>(j, contrib) && valid) atomicAdd(&sda[j], contrib);
+        },
+        [&](int j, int64_t p) { dA[p] = sda[j]; });
+  }
+}
+
+// Rows whose G row is too wide for a table: persistent blocks take
+// (row, column chunk) tickets like the forward's dense bin; G's chunk is
+// scattered into a dense LDS array, products come from the column-blocked
+// B and reach B's original slot through perm.
+template <int T, typename V>
+__global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const int64_t* bptr, const uint16_t* bcol,
+                                                 const V* bval, const int64_t* perm, const int32_t* list,
+                                                 const int32_t* counts, int bin, int R, int W, Probs pr,
+                                                 unsigned long long* ticket, V* dA, V* dB) {
+  extern __shared__ __align__(16) unsigned char dyn_raw[];
+  V* gd = (V*)dyn_raw;
+  __shared__ ExpandLds<T> L;
+  __shared__ V aval[T];
+  __shared__ V sda[T];
+  __shared__ unsigned long long s_ticket;
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  for (int i = threadIdx.x; i < W; i += T) gd[i] = V(0);
+  const unsigned long long ntask = (unsigned long long)H * (unsigned long long)R;
+  const V* av = (const V*)A.val;
+  const V* gval = (const V*)G.val;
+  const int64_t ld = mB + 1;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1ull);
+    __syncthreads();
+    const unsigned long long tk = s_ticket;
+    if (tk >= ntask) break;
+    const int c = (int)(tk / (unsigned long long)H);
+    const int32_t row = list[start + (int32_t)(tk - (unsigned long long)c * H)];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    const int64_t c0 = lo + (int64_t)c * W, c1 = c0 + W;
+    // G's entries of this chunk: G rows are sorted, so a binary search finds them
+    int64_t e0 = G.rp[row], e1 = G.rp[row + 1];
+    {
+      int64_t a = e0, b = e1;
+      while (a < b) { const int64_t mid = (a + b) >> 1; if (G.col[mid] < c0) a = mid + 1; else b = mid; }
+      e0 = a;
+      b = e1;
+      while (a < b) { const int64_t mid = (a + b) >> 1; if (G.col[mid] < c1) a = mid + 1; else b = mid; }
+      e1 = a;
+    }
+    if (e0 == e1) continue;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += T) gd[G.col[e] - c0] = gval[e];
+    __syncthreads();
+    const int64_t* bp = bptr + (int64_t)c * ld;
+    block_expand_flush<T>(
+        A.rp[row], A.rp[row + 1], L,
+        [&](int64_t p, int64_t& st, int32_t& len) {
+          const int32_t k = A.col[p];
+          st = bp[k];
+          len = (int32_t)(bp[k + 1] - st);
+          aval[threadIdx.x] = av[p];
+          sda[threadIdx.x] = V(0);
+        },
+        [&](bool valid, int j, int64_t, int64_t q) {
+          V contrib = V(0);
+          if (valid) {
+            const V g = gd[bcol[q]];
+            unsafeAtomicAdd(&dB[perm[q]], aval[j] * g);
+            contrib = g * bval[q];
+          }
+          if (wave_run_sum<V>(j, contrib) && valid) atomicAdd(&sda[j], contrib);
+        },
+        [&](int j, int64_t p) {
+          if (sda[j] != V(0)) unsafeAtomicAdd(&dA[p], sda[j]);
+        });
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += T) gd[G.col[e] - c0] = V(0);
+  }
+}
+
+// Backward route of every row: by nnz of G's row.
+__global__ __launch_bounds__(256) void k_classify_bwd(int64_t m, const int64_t* g_rp, int64_t hash_max,
+                                                      uint8_t* bin, int32_t* counts) {
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    int b = NB_NONE;
+    if (i < m) {
+      const int64_t z = g_rp[i + 1] - g_rp[i];
+      if (z > 0) {
+        if (z > hash_max) {
+          b = NB_DENSE;
+        } else {
+          const int64_t s = next_pow2_i64(z + ((z + 1) >> 1));
+          b = s <= 64 ? 0 : ilog2_pow2(s) - 6;
+        }
+      }
+      bin[i] = (uint8_t)b;
+    }
+    (void)wave_append(counts, b, b != NB_NONE);
+  }
+}
+
+}  // namespace ams

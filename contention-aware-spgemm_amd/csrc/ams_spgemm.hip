@@ -19,6 +19,7 @@
 
 #include "ams_hip.h"
 #include "ams_kernels_impl.hpp"
+#include "ams_bwd_impl.hpp"
 
 using namespace ams;
 
@@ -119,7 +120,7 @@ static int grow(ams_ctx_t* ctx, Buf& b, size_t bytes) {
     b.p = nullptr;
     b.cap = 0;
   }
-  const size_t want = bytes + bytes / 8;
+  const size_t want = bytes < ((size_t)1 << 30) ? bytes + bytes / 8 : bytes;  // no slack on huge buffers
   hipError_t e = hipMalloc(&b.p, want);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -542,6 +543,82 @@ static Probs single_probs(const ams_dcsr_t* b) {
   pr.width0 = b->num_col;
   pr.max_width = b->num_col;
   return pr;
+}
+
+// ------------------------------------------------------------- backward
+
+template <int S, typename V>
+static void launch_bwd_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Csr& G, V* dA, V* dB, int bin) {
+  constexpr int T = hash_threads<S>();
+  const int64_t cnt = ctx->cnt_num[bin];
+  if (cnt == 0) return;
+  int occ;
+  occupancy(k_bwd_hash<S, T, V>, T, 0, &occ);
+  const int g = grid_for(ctx, occ, cnt, true);
+  Ctrl* c = dctrl(ctx);
+  hipLaunchKernelGGL((k_bwd_hash<S, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B, G, (const int32_t*)ctx->lnum.p,
+                     (const int32_t*)c->cnt_num, bin, dA, dB, &c->err);
+}
+
+template <typename V>
+static int masked_bwd_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, const ams_dcsr_t* g, V* dA,
+                        V* dB) {
+  const Csr A = to_csr(a), B = to_csr(b), G = to_csr(g);
+  const int64_t m = A.m;
+  ctx->plan_valid = false;  // the routing buffers are reused below
+  CHK(grow(ctx, ctx->ctrl, sizeof(Ctrl)));
+  CHK(grow(ctx, ctx->nbin, std::max<int64_t>(m, 1)));
+  CHK(grow(ctx, ctx->lnum, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  Ctrl* c = dctrl(ctx);
+  CHK(fill_i32(ctx, (int32_t*)c, sizeof(Ctrl) / 4, 0));
+  if (sizeof(V) == 8) {
+    CHK(fill_i64(ctx, (int64_t*)dA, A.nnz, 0));
+    CHK(fill_i64(ctx, (int64_t*)dB, B.nnz, 0));
+  } else {
+    CHK(fill_i32(ctx, (int32_t*)dA, A.nnz, 0));
+    CHK(fill_i32(ctx, (int32_t*)dB, B.nnz, 0));
+  }
+  if (m == 0) return AMS_OK;
+  const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
+  hipLaunchKernelGGL(k_classify_bwd, dim3(gc), dim3(256), 0, ctx->stream, m, G.rp, (int64_t)ctx->prm.num_hash_max,
+                     (uint8_t*)ctx->nbin.p, c->cnt_num);
+  hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->nbin.p, kNumBins,
+                     (const int32_t*)c->cnt_num, c->cur_num, (int32_t*)ctx->lnum.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ctx->h_ctrl, c, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::memcpy(ctx->cnt_num, ctx->h_ctrl->cnt_num, sizeof(ctx->cnt_num));
+  launch_bwd_hash<64, V>(ctx, A, B, G, dA, dB, 0);
+  launch_bwd_hash<128, V>(ctx, A, B, G, dA, dB, 1);
+  launch_bwd_hash<256, V>(ctx, A, B, G, dA, dB, 2);
+  launch_bwd_hash<512, V>(ctx, A, B, G, dA, dB, 3);
+  launch_bwd_hash<1024, V>(ctx, A, B, G, dA, dB, 4);
+  launch_bwd_hash<2048, V>(ctx, A, B, G, dA, dB, 5);
+  launch_bwd_hash<4096, V>(ctx, A, B, G, dA, dB, 6);
+  launch_bwd_hash<8192, V>(ctx, A, B, G, dA, dB, 7);
+  HIPCHK(hipGetLastError());
+  if (ctx->cnt_num[NB_DENSE] > 0) {
+    const Probs pr = single_probs(b);
+    int W, R, Wb;
+    dense_geometry(ctx, sizeof(V) == 8 ? AMS_F64 : AMS_F32, pr.max_width, W, R, Wb);
+    CHK(build_bblk<V>(ctx, B, pr, W, R, true));
+    const size_t dyn = (size_t)W * sizeof(V);
+    HIPCHK(hipFuncSetAttribute((const void*)k_bwd_dense<kDenseT, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dyn));
+    CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
+    int occ;
+    occupancy(k_bwd_dense<kDenseT, V>, kDenseT, dyn, &occ);
+    const int g = grid_for(ctx, occ, (int64_t)ctx->cnt_num[NB_DENSE] * R, true);
+    hipLaunchKernelGGL((k_bwd_dense<kDenseT, V>), dim3(g), dim3(kDenseT), dyn, ctx->stream, A, G, B.m,
+                       (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
+                       (const int64_t*)ctx->bperm.p, (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num,
+                       (int)NB_DENSE, R, W, pr, &c->ticket, dA, dB);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(&ctx->h_ctrl->err, &c->err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "backward kernels raised error flags 0x%x", ctx->h_ctrl->err);
+  return AMS_OK;
 }
 
 // ----------------------------------------------------------------- C ABI
@@ -991,9 +1068,15 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   return AMS_OK;
 }
 
-int ams_spgemm_masked_bwd(ams_ctx_t* ctx, const ams_dcsr_t*, const ams_dcsr_t*, const ams_dcsr_t*, int, void*,
-                          void*) {
-  return fail(ctx, AMS_ERR_STATE, "masked backward is not built yet");
+int ams_spgemm_masked_bwd(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, const ams_dcsr_t* g, int dtype,
+                          void* da, void* db) {
+  CHK(check_pair(ctx, a, b, dtype));
+  CHK(check_csr(ctx, g, "G"));
+  if (g->num_row != a->num_row || g->num_col != b->num_col)
+    return fail(ctx, AMS_ERR_ARG, "G must be num_row(A) x num_col(B)");
+  if ((a->num_nnz > 0 && !da) || (b->num_nnz > 0 && !db)) return fail(ctx, AMS_ERR_ARG, "NULL gradient buffer");
+  if (dtype == AMS_F64) return masked_bwd_t<double>(ctx, a, b, g, (double*)da, (double*)db);
+  return masked_bwd_t<float>(ctx, a, b, g, (float*)da, (float*)db);
 }
 
 }  // extern "C"

@@ -1,0 +1,66 @@
+"""Batched form: a stacked batch replayed from one captured hipGraph must
+give every problem's product as the oracle does, on every replay."""
+import numpy as np
+import pytest
+
+from ams_hip import gen
+from helpers import assert_matches_oracle, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(count, seed):
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(count):
+        s = int(rng.integers(4, 10))
+        ef = int(rng.choice([2, 4, 8, 16]))
+        a = gen.rmat_symmetric(s, ef, seed=seed * 1000 + i, values="uniform")
+        probs.append((a, a))
+    probs.append((random_csr(7, 11, 0.3, seed=1), random_csr(11, 300, 0.05, seed=2)))
+    z = random_csr(5, 6, 0.0, seed=3)
+    probs.append((z, random_csr(6, 9, 0.5, seed=4)))
+    return probs
+
+
+@pytest.mark.parametrize("params", [{}, dict(sym_hash_max=16, num_hash_max=16, bitmap_ratio=0.0,
+                                              dense_width=64, bitmap_bits=256, dense_density=0.0)],
+                         ids=["default", "all_dense"])
+def test_batched_graph_replays(gpu_ctx, params):
+    import torch
+    from ams_hip.batched import BatchedProduct
+    from ams_hip.device import Context
+    probs = small_batch(40, seed=5)
+    stream = torch.cuda.Stream()
+    ctx = Context(stream=stream)
+    if params:
+        ctx.set_params(**params)
+    bp = BatchedProduct(probs, ams=ctx)
+    for rep in range(3):
+        with torch.cuda.stream(stream):
+            bp.c_val.fill_(float("nan"))
+            bp.run()
+        res = bp.results()
+        assert int(bp.nnz.cpu()[0]) == bp.cap
+        for (a, b), c in zip(probs, res):
+            assert c.sorted_rows()
+            assert_matches_oracle(c, a, b, exact=False)
+    bp.close()
+    ctx.close()
+
+
+def test_batched_matches_plain(gpu_ctx):
+    import torch
+    from ams_hip.batched import BatchedProduct
+    from ams_hip.device import DeviceCsr
+    probs = small_batch(20, seed=9)
+    bp = BatchedProduct(probs)
+    bp.run()
+    res = bp.results()
+    for (a, b), c in zip(probs, res):
+        p = gpu_ctx.spgemm(DeviceCsr.from_host(a), DeviceCsr.from_host(b)).to_host()
+        np.testing.assert_array_equal(c.row_ptrs, p.row_ptrs)
+        np.testing.assert_array_equal(c.cols, p.cols)
+        np.testing.assert_allclose(c.rats, p.rats, rtol=1e-12, atol=1e-13)
+    torch.cuda.synchronize()
+    bp.close()
