@@ -144,6 +144,90 @@ def cpu_baseline(a, b, target_s, threads):
                       f"AMS_MIN_NUM 32768, DN_MIN 256"}
 
 
+def bench_batched(count, seed, reps):
+    """The uneven batch of skewed R-MAT products (scale U{8..14}, edge factor
+    U{2,4,8,16,32}, values U[-1,1]) as one captured graph, against the same
+    products launched one by one through the one-shot plain entry point."""
+    import torch
+
+    from ams_hip import gen
+    from ams_hip.batched import BatchedProduct
+    from ams_hip.device import Context, DeviceCsr
+
+    t0 = time.perf_counter()
+    probs = gen.batch_problems(count, seed=seed)
+    log(f"batch of {count} generated in {time.perf_counter() - t0:.1f} s")
+    P = sum(int((np.diff(a.row_ptrs).astype(np.int64) ** 2).sum()) for a in probs)
+    stream = torch.cuda.Stream()
+    ctx = Context(stream=stream)
+    bp = BatchedProduct([(a, a) for a in probs], ams=ctx)
+    bp.run()
+    bp.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bp.run()
+    bp.synchronize()
+    t_batched = (time.perf_counter() - t0) / reps
+    nnz_batched = int(bp.nnz.cpu()[0])
+    bp.close()
+    plain = Context()
+    dev = [DeviceCsr.from_host(a) for a in probs]
+    torch.cuda.synchronize()
+    for d in dev[:8]:
+        plain.spgemm_pooled(d, d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nnz_plain = 0
+    for d in dev:
+        c = plain.spgemm_pooled(d, d)
+        nnz_plain += c.num_nnz
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain.close()
+    ctx.close()
+    return {"problems": count, "flops_P": P, "nnz_c": nnz_batched, "nnz_c_plain_sum": nnz_plain,
+            "batched_ms": t_batched * 1e3, "plain_sum_ms": t_plain * 1e3,
+            "speedup_vs_plain_launches": t_plain / t_batched,
+            "batched_gflops": 2.0 * P / t_batched / 1e9, "graph_replays": reps}
+
+
+def bench_fwd_bwd(scale, ef, reps):
+    """Differentiable product C = A * B, B = A^T, FP32 values N(0,1): the
+    forward against forward + masked-SpGEMM backward with G ~ N(0,1) on
+    pattern(C)."""
+    import torch
+
+    from ams_hip import gen
+    from ams_hip.device import Context, DeviceCsr
+
+    a = gen.rmat_symmetric(scale, ef, gen.G500, seed=SEED + 5, values="normal", val_dtype=np.float32)
+    b = a.transpose()
+    A, B = DeviceCsr.from_host(a), DeviceCsr.from_host(b)
+    ctx = Context()
+    C = ctx.spgemm(A, B)
+    g = DeviceCsr(C.num_row, C.num_col, C.row_ptr, C.col,
+                  torch.randn(C.num_nnz, dtype=torch.float32, device=C.val.device))
+    P = int((np.diff(a.row_ptrs).astype(np.int64) ** 2).sum())
+    ctx.spgemm_pooled(A, B)
+    ctx.masked_bwd(A, B, g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.spgemm_pooled(A, B)
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.spgemm_pooled(A, B)
+        ctx.masked_bwd(A, B, g)
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / reps
+    ctx.close()
+    return {"workload": f"G500 R-MAT s{scale} ef{ef}, FP32 N(0,1), C = A*B with B = A^T", "flops_P": P,
+            "nnz_c": C.num_nnz, "fwd_ms": t_f * 1e3, "fwd_bwd_ms": t_fb * 1e3, "fwd_bwd_over_fwd": t_fb / t_f,
+            "fwd_gflops": 2.0 * P / t_f / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,8 +236,21 @@ def main():
     ap.add_argument("--workload", default="g500_s19_ef32", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and fwd+bwd sections")
     ap.add_argument("--param", action="append", default=[], help="routing parameter key=value")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is a design "
               "(row blocks split by flops), not a run.", file=sys.stderr)
@@ -213,6 +310,7 @@ def main():
                 "kernel": kernel, "alg_bytes_per_launch": bytes_dom, "avg_launch_ms": ms_dom / n_dom}
     wb = whole_bytes(st, v, a.num_nnz, a.num_row)
 
+    params_used = ctx.params()
     ver = ctx.verify(c, 0)
     checks = {"nnz_c": ver["nnz"], "sum_col": ver["sum_col"], "sum_val": ver["sum_val"],
               "unsorted_rows": ver["unsorted_rows"], "sum_val_equals_P": ver["sum_val"] == float(P)}
@@ -223,13 +321,24 @@ def main():
         log(f"cpu baseline on {threads} threads, target {args.cpu_seconds} s")
         cpu = cpu_baseline(a, b, args.cpu_seconds, threads)
 
+    extras = {}
+    if not args.no_extras:
+        del ctx
+        torch.cuda.synchronize()
+        log("batched section")
+        extras["batched"] = bench_batched(512, SEED, 3)
+        log(f"batched: {extras['batched']}")
+        log("fwd+bwd section")
+        extras["fwd_bwd"] = bench_fwd_bwd(18, 16, 3)
+        log(f"fwd+bwd: {extras['fwd_bwd']}")
+
     line = {
         "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded R-MAT / incidence generator)",
         "config": {"workload": WORKLOADS[args.workload]["text"], "name": args.workload, "m": a.num_row,
                    "n": b.num_col, "nnz_a": a.num_nnz, "flops_P": P, "nnz_c": st["nnz_c"],
-                   "params": ctx.params()},
+                   "params": params_used},
         "roofline": roof,
         "pipeline": {"alg_bytes_per_step": wb, "frac_hbm_roofline": wb / (ms_step * 1e-3) / HBM_PEAK,
                      "phase_ms_per_step": {k: ms / args.steps for k, ms in phases.items()},
@@ -237,6 +346,7 @@ def main():
                                                    "rows_num_esc", "rows_num_dense", "dense_chunks")}},
         "checks": checks,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(line), flush=True)
 

@@ -44,6 +44,8 @@ def rmat_undirected_edges(scale, ef, abc=G500, seed=0):
     a, b, c = abc
     n = 1 << scale
     target = ef * n
+    if target > n * (n - 1) // 2:
+        raise ValueError(f"scale {scale} has only {n * (n - 1) // 2} loop-free edges, {target} asked")
     rng = np.random.default_rng(seed)
     keys = np.empty(0, np.int64)
     batch = target + target // 4 + 1024
@@ -93,40 +95,46 @@ def rmat_symmetric(scale, ef, abc=G500, seed=0, values="ones", val_dtype=np.floa
 def join_incidence(num_row=850_000, num_col=20_000, mu=2.0, sigma=1.5, cap=5000,
                    zipf_s=1.2, seed=0):
     """Rows: lognormal degrees clipped to [1, cap]; columns: Zipf(zipf_s) ids
-    (rank r -> column r - 1) without repeats inside a row; values 1.0."""
+    (rank r -> column r - 1) without repeats inside a row; values 1.0.
+
+    Each row keeps its first `degree` distinct draws in draw order; rows
+    short of their degree after a round draw again (more each round)."""
     rng = np.random.default_rng(seed)
     deg = np.clip(np.rint(rng.lognormal(mu, sigma, num_row)), 1, cap).astype(np.int64)
     pmf = np.arange(1, num_col + 1, dtype=np.float64) ** (-zipf_s)
     cdf = np.cumsum(pmf)
     cdf /= cdf[-1]
-    rows = np.empty(0, np.int64)
-    cols = np.empty(0, np.int64)
-    need = deg.copy()
-    for _ in range(200):
-        todo = np.nonzero(need > 0)[0]
+    done = []                                # final keys (row * num_col + col) of finished rows
+    todo = np.arange(num_row, dtype=np.int64)
+    cur = np.empty(0, np.int64)              # partial keys of the rows in todo, draw order
+    factor = 1.25
+    for _ in range(64):
         if len(todo) == 0:
             break
-        draw = np.repeat(todo, need[todo] * 2 + 4)
-        c = np.searchsorted(cdf, rng.random(len(draw)), side="right")
-        c = np.minimum(c, num_col - 1)
-        key = np.concatenate([rows * num_col + cols, draw * num_col + c])
-        # keep the first occurrence of each (row, col), in draw order
-        _, first = np.unique(key, return_index=True)
+        have = np.bincount(cur // num_col, minlength=num_row)[todo] if len(cur) else np.zeros(len(todo), np.int64)
+        want = ((deg[todo] - have) * factor).astype(np.int64) + 4
+        rows = np.repeat(todo, want)
+        cols = np.minimum(np.searchsorted(cdf, rng.random(len(rows)), side="right"), num_col - 1)
+        keys = np.concatenate([cur, rows * num_col + cols])
+        _, first = np.unique(keys, return_index=True)
         first.sort()
-        key = key[first]
-        r, cc = key // num_col, key % num_col
-        # truncate every row to its degree, keeping draw order
-        order = np.argsort(r, kind="stable")
-        r, cc = r[order], cc[order]
-        start = np.searchsorted(r, np.arange(num_row))
-        rank = np.arange(len(r)) - start[r]
-        keep = rank < deg[r]
-        rows, cols = r[keep], cc[keep]
-        need = deg - np.bincount(rows, minlength=num_row)
+        keys = keys[first]                   # distinct, in draw order
+        order = np.argsort(keys // num_col, kind="stable")
+        keys = keys[order]
+        r = keys // num_col
+        start = np.searchsorted(r, r, side="left")
+        keep = (np.arange(len(keys)) - start) < deg[r]
+        keys, r = keys[keep], r[keep]
+        cnt = np.bincount(r, minlength=num_row)
+        full = cnt[r] == deg[r]
+        done.append(keys[full])
+        cur = keys[~full]
+        todo = todo[cnt[todo] < deg[todo]]
+        factor *= 2.0
     else:
         raise RuntimeError("join incidence generator did not converge")
-    order = np.lexsort((cols, rows))
-    rows, cols = rows[order], cols[order]
+    keys = np.sort(np.concatenate(done))
+    rows, cols = keys // num_col, keys % num_col
     rp = np.zeros(num_row + 1, np.int64)
     np.cumsum(np.bincount(rows, minlength=num_row), out=rp[1:])
     return Csr(num_row, num_col, rp, cols.astype(np.int32), np.ones(len(rows), np.float64))

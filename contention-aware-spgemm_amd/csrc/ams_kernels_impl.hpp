@@ -325,9 +325,18 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
                         const uint32_t u = (uint32_t)((int64_t)B.col[q] - base);
                         if (u < (uint32_t)pw) atomicOr(&bm[u >> 5], 1u << (u & 31));
                       });
-      for (int w = threadIdx.x; w < words; w += T) {
-        const int pc = __popc(bm[w]);
-        if (pc) atomicAdd(&ccount[w / wchunk], pc);
+      // words_pass rounded up to whole waves so every lane joins the vote
+      for (int w0 = 0; w0 < words; w0 += T) {
+        const int w = w0 + threadIdx.x;
+        const int pc = w < words ? __popc(bm[w]) : 0;
+        const int cidx = w < words ? w / wchunk : -1;
+        const int c_lane0 = __shfl(cidx, 0, kWave);
+        if (__all(cidx == c_lane0)) {
+          const int tot = wave_sum(pc);
+          if (lane_id() == 0 && tot && c_lane0 >= 0) atomicAdd(&ccount[c_lane0], tot);
+        } else if (pc) {
+          atomicAdd(&ccount[cidx], pc);
+        }
       }
       __syncthreads();
       if (threadIdx.x == 0) {
@@ -542,6 +551,8 @@ __global__ __launch_bounds__(64) void k_bblk_scatter(Csr B, Probs pr, int W, int
 // (chunk-major, so concurrently running morsels read the same column block
 // of B) for heavy row h of the bitmap list and chunk c.  Dynamic LDS:
 //   V dense[W] | uint32 bitmap[W/32] | union{ExpandLds, uint16 idx[W]}
+constexpr unsigned long long kGrab = 8;
+
 template <int T, typename V>
 __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64_t* bptr,
                                                   const uint16_t* bcol, const V* bval,
@@ -568,14 +579,21 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
   const unsigned long long ntask = (unsigned long long)H * (unsigned long long)R;
   const V* av = (const V*)A.val;
   const int64_t ld = mB + 1;
+  // tickets [tk, tk_end) are claimed kGrab at a time to keep the queue
+  // counter cold; every thread holds the same copy
+  unsigned long long tk = 0, tk_end = 0;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1ull);
-    __syncthreads();
-    const unsigned long long tk = s_ticket;
-    if (tk >= ntask) break;
-    const int c = (int)(tk / (unsigned long long)H);
-    const int32_t h = (int32_t)(tk - (unsigned long long)c * H);
+    if (tk >= tk_end) {
+      if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, kGrab);
+      __syncthreads();
+      tk = s_ticket;
+      tk_end = tk + kGrab;
+    }
+    const unsigned long long cur = tk++;
+    if (cur >= ntask) break;
+    const int c = (int)(cur / (unsigned long long)H);
+    const int32_t h = (int32_t)(cur - (unsigned long long)c * H);
     const int32_t row = list[start + h];
     if (nbin[row] != NB_DENSE) continue;
     const int32_t* crow = coff + (int64_t)h * (R + 1);

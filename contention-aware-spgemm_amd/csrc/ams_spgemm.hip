@@ -241,7 +241,8 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
   W = std::max(64, std::min(W, 65536));
   R = (int)((max_width + W - 1) / W);
   if (R < 1) R = 1;
-  int64_t wb = (int64_t)ctx->prm.bitmap_bits / W * W;
+  // one bitmap pass covers the whole row when that fits bitmap_bits
+  int64_t wb = std::min<int64_t>((int64_t)R * W, (int64_t)ctx->prm.bitmap_bits / W * W);
   if (wb < W) wb = W;
   Wb = (int)wb;
   (void)dtype;
@@ -631,7 +632,7 @@ void ams_default_params(ams_params_t* p) {
   p->esc_max = 4096;
   p->acc_thres = 2.0;
   p->dense_width = 7424;
-  p->bitmap_bits = 1 << 19;
+  p->bitmap_bits = 1 << 20;
   p->bitmap_ratio = 1.0;
   p->dense_density = 0.05;
   p->mode = AMS_MODE_HYBRID;

@@ -13,7 +13,7 @@ def small_batch(count, seed):
     rng = np.random.default_rng(seed)
     probs = []
     for i in range(count):
-        s = int(rng.integers(4, 10))
+        s = int(rng.integers(6, 11))
         ef = int(rng.choice([2, 4, 8, 16]))
         a = gen.rmat_symmetric(s, ef, seed=seed * 1000 + i, values="uniform")
         probs.append((a, a))
