@@ -69,20 +69,44 @@ struct ExpandLds {
   int64_t wtmp[T / kWave];
 };
 
+// Largest j in [cur, n) with off[j] <= t, given off[cur] <= t and off
+// non-decreasing: gallop forward from cur, then bisect.  One LDS read when
+// t is still inside segment cur, which is the common case on long B rows.
+__device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, int32_t t) {
+  int lo = cur, hi = n, step = 1;
+  for (;;) {
+    const int nx = lo + step;
+    if (nx >= n) break;
+    if (off[nx] > t) { hi = nx; break; }
+    lo = nx;
+    step <<= 1;
+  }
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // Enumerates the products of A nonzeros [p0, p1): nonzero p contributes the
 // B positions [start(p), start(p) + len(p)).  Windows of T nonzeros are
-// flattened with a block scan; each thread then takes products t = tid,
-// tid + T, ... of the window and finds its nonzero by binary search over the
-// window's prefix.  Consecutive products of one segment go to consecutive
-// lanes, so B reads coalesce whatever the segment lengths are.
-//   seg(p, start&, len&)          fills the segment of nonzero p (p < p1);
-//                                 it runs on thread p - window base
-//   body(j, p, q, t)              j = index in window, q = B position,
-//                                 t = rank of the product in A-row order
+// flattened with a block scan.  In every T*U-wide stride of the window,
+// wave w owns the 64*U consecutive products starting at w*64*U and lane l
+// takes products l, l+64, ... of them, so consecutive products of one B row
+// go to consecutive lanes and B reads coalesce whatever the segment lengths
+// are, while each lane's cursor moves only 64 products per step.  Each
+// thread keeps the segment of its last product and seeks forward from it.
+// The U products' global reads (load) are issued before their LDS updates
+// (apply), giving every lane U independent loads in flight.
+//   seg(p, start&, len&)     segment of nonzero p (p < p1), on thread p - w0
+//   load(j, p, q, t) -> item j = index in window, q = B position,
+//                            t = rank of the product in A-row order
+//   apply(item)
 // All threads of the block must call it (it holds barriers).
-template <int T, class Seg, class Body>
-__device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg,
-                                             Body body) {
+template <int T, int U, class Seg, class Load, class Apply>
+__device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
+                                             Apply apply) {
+  using Item = decltype(load(0, int64_t(0), int64_t(0), int64_t(0)));
   int64_t base = 0;
   for (int64_t w0 = p0; w0 < p1; w0 += T) {
     const int64_t p = w0 + threadIdx.x;
@@ -96,13 +120,22 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
     __syncthreads();
     const int32_t total = (int32_t)total64;
     const int nvalid = (int)((p1 - w0) < T ? (p1 - w0) : T);
-    for (int32_t t = threadIdx.x; t < total; t += T) {
-      int lo = 0, hi = nvalid;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (L.off[mid] <= t) lo = mid; else hi = mid;
+    int cur = 0;
+    for (int32_t t0 = 0; t0 < total; t0 += T * U) {
+      Item item[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
+        ok[u] = t < total;
+        if (ok[u]) {
+          cur = seek_segment(L.off, nvalid, cur, t);
+          item[u] = load(cur, w0 + cur, L.start[cur] + (int64_t)(t - L.off[cur]), base + t);
+        }
       }
-      body(lo, w0 + lo, L.start[lo] + (int64_t)(t - L.off[lo]), base + t);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ok[u]) apply(item[u]);
     }
     base += total;
     __syncthreads();
@@ -179,6 +212,88 @@ __device__ __forceinline__ void block_bitonic_sort(K* s, int n) {
         if ((x > y) == up) { s[a] = y; s[b] = x; }
       }
       __syncthreads();
+    }
+  }
+}
+
+// Sorts perm[0..n) = 0..n-1 by the composite (key[i], i) ascending: a
+// counting pass over nb (power of two, >= n) order-preserving buckets of
+// the key range [kmin, kmin + span), a scatter, and an insertion sort of
+// every bucket by its owner thread.  Keys spread evenly over the range
+// (rows of uniform graphs) leave O(1) elements per bucket; when a bucket
+// holds more than kMaxBucket elements the whole array is re-sorted with an
+// indirect bitonic sort instead.  cnt holds nb ints.  Ends with a barrier.
+constexpr int kMaxBucket = 32;
+
+template <int T>
+__device__ __forceinline__ void block_sort_perm(const uint32_t* key, uint16_t* perm, int32_t* cnt, int n, int nb,
+                                                uint32_t kmin, uint64_t span, int32_t* s_flag) {
+  // monotone bucket map: float conversion, scaling and truncation all keep
+  // order, so bucket(k1) <= bucket(k2) whenever k1 <= k2
+  const float scale = (float)nb / (float)span;
+  auto bucket = [&](uint32_t k) -> int {
+    const int b = (int)((float)(k - kmin) * scale);
+    return b < nb ? b : nb - 1;
+  };
+  for (int b = threadIdx.x; b < nb; b += T) cnt[b] = 0;
+  if (threadIdx.x == 0) *s_flag = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += T) atomicAdd(&cnt[bucket(key[i])], 1);
+  __syncthreads();
+  // exclusive scan of cnt: each thread owns nb / T consecutive buckets
+  __shared__ int32_t s_w[T / kWave];
+  const int per = nb >= T ? nb / T : 1;
+  const int b0 = threadIdx.x * per;
+  int32_t mine = 0;
+  if (b0 < nb)
+    for (int b = b0; b < b0 + per; ++b) mine += cnt[b];
+  int32_t tot;
+  int32_t run = block_excl_scan<T, int32_t>(mine, s_w, tot);
+  if (b0 < nb)
+    for (int b = b0; b < b0 + per; ++b) {
+      const int32_t c = cnt[b];
+      cnt[b] = run;
+      run += c;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += T) perm[atomicAdd(&cnt[bucket(key[i])], 1)] = (uint16_t)i;
+  __syncthreads();
+  // cnt[b] is now the end of bucket b
+  for (int b = threadIdx.x; b < nb; b += T) {
+    const int e = cnt[b], s = b == 0 ? 0 : cnt[b - 1];
+    if (e - s > kMaxBucket) {
+      *s_flag = 1;
+      continue;
+    }
+    for (int x = s + 1; x < e; ++x) {
+      const uint16_t v = perm[x];
+      const uint64_t kv = ((uint64_t)key[v] << 16) | v;
+      int y = x - 1;
+      while (y >= s && (((uint64_t)key[perm[y]] << 16) | perm[y]) > kv) {
+        perm[y + 1] = perm[y];
+        --y;
+      }
+      perm[y + 1] = v;
+    }
+  }
+  __syncthreads();
+  if (*s_flag) {
+    const int n2 = 1 << (32 - __clz(n - 1 > 0 ? n - 1 : 1));
+    for (int i = threadIdx.x; i < n2; i += T) perm[i] = (uint16_t)i;  // identity, padded
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < (n2 >> 1); i += T) {
+          const int a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+          const int c = a + j;
+          const uint16_t x = perm[a], y = perm[c];
+          const uint64_t kx = x < n ? (((uint64_t)key[x] << 16) | x) : ~0ull;
+          const uint64_t ky = y < n ? (((uint64_t)key[y] << 16) | y) : ~0ull;
+          const bool up = (a & k) == 0;
+          if ((kx > ky) == up) { perm[a] = y; perm[c] = x; }
+        }
+        __syncthreads();
+      }
     }
   }
 }

@@ -70,6 +70,19 @@ struct SegB {
   }
 };
 
+template <typename V>
+struct KeyVal {
+  int32_t key;
+  V val;
+};
+
+template <typename V>
+struct RankKeyVal {
+  int32_t rank;  // product's rank in A-row order
+  int32_t key;   // column
+  V val;
+};
+
 // ------------------------------------------------------------- row flops
 
 // One wave per row of A; also accumulates P into *total.
@@ -92,6 +105,42 @@ __global__ __launch_bounds__(256) void k_row_flops(Csr A, Csr B, int64_t* row_fl
     }
   }
   const int64_t t = block_sum<256, int64_t>(acc, s_w);
+  if (threadIdx.x == 0 && t) atomicAdd(total, (unsigned long long)t);
+}
+
+// Nonzero-split row flops for matrices with very long rows: each thread
+// takes kFlpIpt consecutive nonzeros of A, finds its first row by binary
+// search over A.rp, and adds its partial sums to row_flp with one atomic per
+// row it touches.  row_flp must start at zero.
+constexpr int kFlpIpt = 8;
+__global__ __launch_bounds__(256) void k_row_flops_nz(Csr A, Csr B, unsigned long long* row_flp,
+                                                      unsigned long long* total) {
+  __shared__ int64_t s_w[4];
+  int64_t acc_all = 0;
+  const int64_t span = (int64_t)gridDim.x * blockDim.x * kFlpIpt;
+  for (int64_t z0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kFlpIpt; z0 < A.nnz; z0 += span) {
+    int64_t lo = 0, hi = A.m;  // largest r with rp[r] <= z0
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (A.rp[mid] <= z0) lo = mid; else hi = mid;
+    }
+    int64_t r = lo, rend = A.rp[r + 1], acc = 0;
+    const int64_t z1 = z0 + kFlpIpt < A.nnz ? z0 + kFlpIpt : A.nnz;
+    for (int64_t p = z0; p < z1; ++p) {
+      while (p >= rend) {
+        if (acc) atomicAdd(&row_flp[r], (unsigned long long)acc);
+        acc = 0;
+        ++r;
+        rend = A.rp[r + 1];
+      }
+      const int32_t k = A.col[p];
+      const int64_t d = B.rp[k + 1] - B.rp[k];
+      acc += d;
+      acc_all += d;
+    }
+    if (acc) atomicAdd(&row_flp[r], (unsigned long long)acc);
+  }
+  const int64_t t = block_sum<256, int64_t>(acc_all, s_w);
   if (threadIdx.x == 0 && t) atomicAdd(total, (unsigned long long)t);
 }
 
@@ -279,12 +328,13 @@ __global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* li
     for (int s = threadIdx.x; s < S; s += T) keys[s] = kEmpty;
     __syncthreads();
     int64_t mine = 0;
-    block_expand<T>(A.rp[row], A.rp[row + 1], L, seg,
-                    [&](int, int64_t, int64_t q, int64_t) {
-                      const int r = lds_insert_key(keys, shift, B.col[q]);
-                      if (r > 0) ++mine;
-                      if (r < 0) atomicOr(err, 8);
-                    });
+    block_expand<T, 2>(A.rp[row], A.rp[row + 1], L, seg,
+                       [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+                       [&](int32_t key) {
+                         const int r = lds_insert_key(keys, shift, key);
+                         if (r > 0) ++mine;
+                         if (r < 0) atomicOr(err, 8);
+                       });
     const int64_t tot = block_sum<T, int64_t>(mine, s_w);
     if (threadIdx.x == 0) rownz[row] = tot;
   }
@@ -320,11 +370,11 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
-      block_expand<T>(A.rp[row], A.rp[row + 1], L, seg,
-                      [&](int, int64_t, int64_t q, int64_t) {
-                        const uint32_t u = (uint32_t)((int64_t)B.col[q] - base);
-                        if (u < (uint32_t)pw) atomicOr(&bm[u >> 5], 1u << (u & 31));
-                      });
+      block_expand<T, 4>(A.rp[row], A.rp[row + 1], L, seg,
+                         [&](int, int64_t, int64_t q, int64_t) { return (uint32_t)((int64_t)B.col[q] - base); },
+                         [&](uint32_t u) {
+                           if (u < (uint32_t)pw) atomicOr(&bm[u >> 5], 1u << (u & 31));
+                         });
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {
         const int w = w0 + threadIdx.x;
@@ -385,7 +435,7 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
     for (int s = threadIdx.x; s < S; s += T) { keys[s] = kEmpty; vals[s] = V(0); }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    block_expand<T>(A.rp[row], A.rp[row + 1], L,
+    block_expand<T, 4>(A.rp[row], A.rp[row + 1], L,
                     [&](int64_t p, int64_t& st, int32_t& len) {
                       const int32_t k = A.col[p];
                       st = B.rp[k];
@@ -393,8 +443,10 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
                       aval[threadIdx.x] = av[p];
                     },
                     [&](int j, int64_t, int64_t q, int64_t) {
-                      if (!lds_insert_add<V>(keys, vals, shift, B.col[q], aval[j] * bv[q]))
-                        atomicOr(err, 16);
+                      return KeyVal<V>{B.col[q], aval[j] * bv[q]};
+                    },
+                    [&](KeyVal<V> kv) {
+                      if (!lds_insert_add<V>(keys, vals, shift, kv.key, kv.val)) atomicOr(err, 16);
                     });
     for (int s0 = 0; s0 < S; s0 += T) {
       const int s = s0 + threadIdx.x;
@@ -422,62 +474,112 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
   }
 }
 
-// ESC of one row per block: expand the row's f <= E products into LDS as
-// (column << 32 | rank) with their values, bitonic-sort, then every run of
-// equal columns is summed left to right by the thread that owns its head.
-// The rank keeps equal columns in A-row order, so each value is the same
-// left-to-right sum of separately rounded products as the reference's.
+// ESC of one row per block: expand the row's f <= E products into LDS
+// (column and value, indexed by the product's rank in A-row order), sort the
+// ranks by (column, rank), then every run of equal columns is summed left to
+// right by the thread that owns its head.  The rank keeps equal columns in
+// A-row order, so each value is the same left-to-right sum of separately
+// rounded products as the reference's.  The sort is a bitonic sort of
+// (column << 32 | rank) words (esc_sort = 0) or a bucket sort of a rank
+// permutation (esc_sort = 1); both share one LDS region laid out as
+//   bucket:  cnt int32[E] | key uint32[E] | perm uint16[E]
+//   bitonic: sk uint64[E] (over cnt and key)   | key copy kept in regs
 template <int E, int T, typename V>
 __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* list,
                                                 const int32_t* counts, int bin, const int64_t* row_flp,
                                                 const int64_t* rp_c, int32_t* col_c, V* val_c,
-                                                int64_t cap, int32_t* err) {
-  __shared__ uint64_t sk[E];
+                                                int64_t cap, int esc_sort, int32_t* err) {
+  static_assert(E <= 65536, "ranks are uint16");
+  constexpr int IPT = (E + T - 1) / T;
+  __shared__ __align__(16) unsigned char region[E * 10];
+  int32_t* cnt = (int32_t*)region;
+  uint32_t* key = (uint32_t*)(region + 4 * E);
+  uint16_t* perm = (uint16_t*)(region + 8 * E);
+  uint64_t* sk = (uint64_t*)region;
   __shared__ V vals[E];
   __shared__ ExpandLds<T> L;
   __shared__ V aval[T];
   __shared__ int64_t s_w[T / kWave];
+  __shared__ uint32_t s_min[T / kWave], s_max[T / kWave];
+  __shared__ int32_t s_flag;
   if (rp_c[A.m] > cap) return;
-  int32_t start, cnt;
-  bin_range(counts, bin, start, cnt);
+  int32_t start, cnt_rows;
+  bin_range(counts, bin, start, cnt_rows);
   const V* av = (const V*)A.val;
   const V* bv = (const V*)B.val;
-  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+  for (int32_t idx = blockIdx.x; idx < cnt_rows; idx += gridDim.x) {
     const int32_t row = list[start + idx];
     const int32_t f = (int32_t)row_flp[row];
     const int64_t off = rp_c[row];
     const int32_t z = (int32_t)(rp_c[row + 1] - off);
-    block_expand<T>(A.rp[row], A.rp[row + 1], L,
-                    [&](int64_t p, int64_t& st, int32_t& len) {
-                      const int32_t k = A.col[p];
-                      st = B.rp[k];
-                      len = (int32_t)(B.rp[k + 1] - st);
-                      aval[threadIdx.x] = av[p];
-                    },
-                    [&](int j, int64_t, int64_t q, int64_t t) {
-                      sk[t] = ((uint64_t)(uint32_t)B.col[q] << 32) | (uint64_t)t;
-                      vals[t] = aval[j] * bv[q];
-                    });
-    const int32_t n = (int32_t)next_pow2_i64(f);
-    for (int i = f + threadIdx.x; i < n; i += T) sk[i] = ~0ull;
-    __syncthreads();
-    block_bitonic_sort<T, uint64_t>(sk, n);
-    const int32_t ipt = n >= T ? n / T : 1;
+    block_expand<T, 4>(A.rp[row], A.rp[row + 1], L,
+                       [&](int64_t p, int64_t& st, int32_t& len) {
+                         const int32_t k = A.col[p];
+                         st = B.rp[k];
+                         len = (int32_t)(B.rp[k + 1] - st);
+                         aval[threadIdx.x] = av[p];
+                       },
+                       [&](int j, int64_t, int64_t q, int64_t t) {
+                         return RankKeyVal<V>{(int32_t)t, B.col[q], aval[j] * bv[q]};
+                       },
+                       [&](RankKeyVal<V> r) {
+                         key[r.rank] = (uint32_t)r.key;
+                         vals[r.rank] = r.val;
+                       });
+    const int nb = (int)next_pow2_i64(f);
+    if (esc_sort == 0) {
+      uint32_t kr[IPT];
+#pragma unroll
+      for (int u = 0; u < IPT; ++u) {
+        const int i = threadIdx.x + u * T;
+        kr[u] = i < f ? key[i] : 0u;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < IPT; ++u) {
+        const int i = threadIdx.x + u * T;
+        if (i < nb) sk[i] = i < f ? (((uint64_t)kr[u] << 32) | (uint64_t)i) : ~0ull;
+      }
+      __syncthreads();
+      block_bitonic_sort<T, uint64_t>(sk, nb);
+    } else {
+      uint32_t kmin = 0xffffffffu, kmax = 0u;
+      for (int i = threadIdx.x; i < f; i += T) {
+        kmin = min(kmin, key[i]);
+        kmax = max(kmax, key[i]);
+      }
+#pragma unroll
+      for (int d = kWave / 2; d > 0; d >>= 1) {
+        kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, d, kWave));
+        kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, d, kWave));
+      }
+      if (lane_id() == 0) { s_min[wave_id()] = kmin; s_max[wave_id()] = kmax; }
+      __syncthreads();
+      kmin = s_min[0];
+      kmax = s_max[0];
+      for (int w = 1; w < T / kWave; ++w) { kmin = min(kmin, s_min[w]); kmax = max(kmax, s_max[w]); }
+      block_sort_perm<T>(key, perm, cnt, f, nb, kmin, (uint64_t)(kmax - kmin) + 1, &s_flag);
+    }
+    // sorted element e: column and rank
+    auto col_at = [&](int e) -> uint32_t { return esc_sort == 0 ? (uint32_t)(sk[e] >> 32) : key[perm[e]]; };
+    auto rank_at = [&](int e) -> uint32_t { return esc_sort == 0 ? (uint32_t)sk[e] : (uint32_t)perm[e]; };
+    const int32_t ipt = nb >= T ? nb / T : 1;
     const int32_t e0 = threadIdx.x * ipt;
     const int32_t e1 = (e0 + ipt) < f ? (e0 + ipt) : f;
     int64_t heads = 0;
-    for (int32_t e = e0; e < e1; ++e)
-      heads += (e == 0) || ((uint32_t)(sk[e] >> 32) != (uint32_t)(sk[e - 1] >> 32));
+    for (int32_t e = e0; e < e1; ++e) heads += (e == 0) || (col_at(e) != col_at(e - 1));
     int64_t tot;
     int64_t rank = block_excl_scan<T, int64_t>(heads, s_w, tot);
     if (threadIdx.x == 0 && tot != z) atomicOr(err, 2);
     for (int32_t e = e0; e < e1; ++e) {
-      const uint32_t key = (uint32_t)(sk[e] >> 32);
-      if (e != 0 && key == (uint32_t)(sk[e - 1] >> 32)) continue;
-      V sum = vals[(uint32_t)sk[e]];
-      for (int32_t g = e + 1; g < f && (uint32_t)(sk[g] >> 32) == key; ++g) sum += vals[(uint32_t)sk[g]];
-      col_c[off + rank] = (int32_t)key;
-      val_c[off + rank] = sum;
+      const uint32_t k = col_at(e);
+      if (e != 0 && k == col_at(e - 1)) continue;
+      V sum = vals[rank_at(e)];
+      for (int32_t g = e + 1; g < f && col_at(g) == k; ++g) sum += vals[rank_at(g)];
+      if (rank < z) {
+        col_c[off + rank] = (int32_t)k;
+        val_c[off + rank] = sum;
+      }
       ++rank;
     }
     __syncthreads();
@@ -579,16 +681,19 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
   const unsigned long long ntask = (unsigned long long)H * (unsigned long long)R;
   const V* av = (const V*)A.val;
   const int64_t ld = mB + 1;
-  // tickets [tk, tk_end) are claimed kGrab at a time to keep the queue
-  // counter cold; every thread holds the same copy
+  // tickets [tk, tk_end) are claimed `grab` at a time to keep the queue
+  // counter cold: up to kGrab when there are many morsels per block, one at
+  // a time when morsels are few (and then often very uneven)
+  const unsigned long long per_block = ntask / ((unsigned long long)gridDim.x * 64ull);
+  const unsigned long long grab = per_block < 1 ? 1 : (per_block > kGrab ? kGrab : per_block);
   unsigned long long tk = 0, tk_end = 0;
   for (;;) {
     __syncthreads();
     if (tk >= tk_end) {
-      if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, kGrab);
+      if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, grab);
       __syncthreads();
       tk = s_ticket;
-      tk_end = tk + kGrab;
+      tk_end = tk + grab;
     }
     const unsigned long long cur = tk++;
     if (cur >= ntask) break;
@@ -604,7 +709,7 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
     prob_cols(pr, row, lo, width);
     const int64_t off = rp_c[row] + cbase;
     const int64_t* bp = bptr + (int64_t)c * ld;
-    block_expand<T>(A.rp[row], A.rp[row + 1], L,
+    block_expand<T, 4>(A.rp[row], A.rp[row + 1], L,
                     [&](int64_t p, int64_t& st, int32_t& len) {
                       const int32_t k = A.col[p];
                       st = bp[k];
@@ -612,9 +717,12 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
                       aval[threadIdx.x] = av[p];
                     },
                     [&](int j, int64_t, int64_t q, int64_t) {
-                      const uint32_t u = bcol[q];
+                      return KeyVal<V>{(int32_t)bcol[q], aval[j] * bval[q]};
+                    },
+                    [&](KeyVal<V> kv) {
+                      const uint32_t u = (uint32_t)kv.key;
                       atomicOr(&bm[u >> 5], 1u << (u & 31));
-                      atomicAdd(&dense[u], aval[j] * bval[q]);
+                      atomicAdd(&dense[u], kv.val);
                     });
     // rank the set bits: each thread owns a run of consecutive words
     const int wpt = (words + T - 1) / T;

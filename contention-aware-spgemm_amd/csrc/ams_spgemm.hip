@@ -253,6 +253,25 @@ static size_t dense_lds(int W, int vbytes) {
   return (size_t)W * vbytes + (size_t)(W / 32) * 4 + ((uni + 15) & ~(size_t)15);
 }
 
+// Row flops: a wave per row, or the nonzero split when rows are long on
+// average (a wave per row would leave most of the GPU idle).
+static int launch_row_flops(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t* row_flp,
+                            unsigned long long* total) {
+  if (A.m == 0) return AMS_OK;
+  if (row_flp && A.nnz > 256 * A.m) {
+    CHK(fill_i64(ctx, row_flp, A.m, 0));
+    const int64_t threads = (A.nnz + kFlpIpt - 1) / kFlpIpt;
+    const int g = (int)std::min<int64_t>((threads + 255) / 256, (int64_t)ctx->num_cu * 32);
+    hipLaunchKernelGGL(k_row_flops_nz, dim3(g), dim3(256), 0, ctx->stream, A, B, (unsigned long long*)row_flp,
+                       total);
+  } else {
+    const int g = (int)std::min<int64_t>((A.m + 3) / 4, (int64_t)ctx->num_cu * 16);
+    hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, row_flp, total);
+  }
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
 // ------------------------------------------------------------ symbolic
 
 template <int S>
@@ -297,9 +316,7 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   if (m > 0) {
     {
       Phase ph(ctx, AMS_PH_FLOPS, sync);
-      const int g = (int)std::min<int64_t>((m + 3) / 4, (int64_t)ctx->num_cu * 16);
-      hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, (int64_t*)ctx->row_flp.p,
-                         &c->flops);
+      CHK(launch_row_flops(ctx, A, B, (int64_t*)ctx->row_flp.p, &c->flops));
     }
     Phase ph(ctx, AMS_PH_CLASSIFY, sync);
     const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
@@ -436,7 +453,7 @@ static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const in
   Ctrl* c = dctrl(ctx);
   hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
                      (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
-                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, &c->err);
+                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
 }
 
 // Column-blocked copy of B (chunk-major CSR of width-W column blocks).
@@ -637,6 +654,7 @@ void ams_default_params(ams_params_t* p) {
   p->dense_density = 0.05;
   p->mode = AMS_MODE_HYBRID;
   p->wgs_per_cu = 0;
+  p->esc_sort = 1;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -702,6 +720,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
     return fail(ctx, AMS_ERR_ARG, "dense_width must be a multiple of 64 in [64, 65536]");
   if (p->bitmap_bits < p->dense_width) return fail(ctx, AMS_ERR_ARG, "bitmap_bits must be >= dense_width");
   if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
+  if (p->esc_sort < 0 || p->esc_sort > 1) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0 or 1");
   ctx->prm = *p;
   return AMS_OK;
 }
@@ -724,11 +743,7 @@ int ams_row_flops(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int6
   Ctrl* c = dctrl(ctx);
   CHK(fill_i64(ctx, (int64_t*)&c->flops, 1, 0));
   const Csr A = to_csr(a), B = to_csr(b);
-  if (A.m > 0) {
-    const int g = (int)std::min<int64_t>((A.m + 3) / 4, (int64_t)ctx->num_cu * 16);
-    hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, row_flp, &c->flops);
-    HIPCHK(hipGetLastError());
-  }
+  CHK(launch_row_flops(ctx, A, B, row_flp, &c->flops));
   if (total) {
     HIPCHK(hipMemcpyAsync(ctx->h_pin, &c->flops, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1060,6 +1075,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   for (double t : thres) { ams_params_t p = best; p.acc_thres = t; consider(p); }
   for (double q : ratios) { ams_params_t p = best; p.bitmap_ratio = q; consider(p); }
   for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
+  for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->err.clear();

@@ -57,6 +57,7 @@ typedef struct {
 	double  dense_density;  /* dense bin also when nnz(C row) >= density * n          */
 	int32_t mode;           /* AMS_MODE_*                                             */
 	int32_t wgs_per_cu;     /* persistent workgroups per CU for the queue kernels     */
+	int32_t esc_sort;       /* ESC row sort in LDS: 0 bitonic, 1 bucket               */
 } ams_params_t;
 
 /* Counters of the last symbolic / numeric call (rows per bin, tasks). */

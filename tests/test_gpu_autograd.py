@@ -100,8 +100,8 @@ def test_autograd_finite_differences_s12(gpu_ctx):
     """FP64 central differences of L = sum(W * C) over 1000 sampled nonzeros
     (500 of A, 500 of B) of a scale-12 skewed R-MAT, B = A^T as its own input.
     L is affine in any single entry of A or B, so the central difference has
-    no truncation error for any step; the step only trades against rounding
-    of L (about 1e-16 * sum |W C|), hence eps = 1e-3."""
+    no truncation error for any step; the step only divides the rounding of
+    L (about log2(nnz) * 1e-16 * sum |W C|, ~1e-8 here), hence a large step."""
     import torch
     from ams_hip import autograd
     from ams_hip.device import DeviceCsr
@@ -126,7 +126,7 @@ def test_autograd_finite_differences_s12(gpu_ctx):
             return float((Cp.val * w).sum())
 
     rng = np.random.default_rng(0)
-    eps = 1e-3
+    eps = 0.5
     worst = 0.0
     for which, n in (("a", A.num_nnz), ("b", B.num_nnz)):
         for q in rng.choice(n, 500, replace=False):
@@ -142,4 +142,4 @@ def test_autograd_finite_differences_s12(gpu_ctx):
             an = float((ga if which == "a" else gb)[q])
             err = abs(fd - an) / max(1.0, abs(an))
             worst = max(worst, err)
-    assert worst < 1e-8, worst
+    assert worst < 1e-7, worst
