@@ -26,7 +26,8 @@ class AmsParams(C.Structure):
     _fields_ = [("sym_hash_max", C.c_int32), ("num_hash_max", C.c_int32), ("esc_max", C.c_int32),
                 ("acc_thres", C.c_double), ("dense_width", C.c_int32), ("bitmap_bits", C.c_int32),
                 ("bitmap_ratio", C.c_double), ("dense_density", C.c_double), ("mode", C.c_int32),
-                ("wgs_per_cu", C.c_int32), ("esc_sort", C.c_int32)]
+                ("wgs_per_cu", C.c_int32), ("esc_sort", C.c_int32),
+                ("sym_blocked", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -100,8 +100,10 @@ __device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, 
 // (apply), giving every lane U independent loads in flight.
 //   seg(p, start&, len&)     segment of nonzero p (p < p1), on thread p - w0
 //   load(j, p, q, t) -> item j = index in window, q = B position,
-//                            t = rank of the product in A-row order
-//   apply(item)
+//                            t = rank of the product in A-row order; it must
+//                            only gather (no arithmetic on loaded values), so
+//                            that the U loads stay in flight together
+//   apply(item)              runs for the lanes whose product exists
 // All threads of the block must call it (it holds barriers).
 template <int T, int U, class Seg, class Load, class Apply>
 __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
@@ -124,14 +126,15 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
     for (int32_t t0 = 0; t0 < total; t0 += T * U) {
       Item item[U];
       bool ok[U];
+      // branch-free issue: lanes past the end re-load the last product and
+      // are masked in apply, so all U loads go out back to back
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
+        int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
         ok[u] = t < total;
-        if (ok[u]) {
-          cur = seek_segment(L.off, nvalid, cur, t);
-          item[u] = load(cur, w0 + cur, L.start[cur] + (int64_t)(t - L.off[cur]), base + t);
-        }
+        t = ok[u] ? t : total - 1;
+        cur = seek_segment(L.off, nvalid, cur, t);
+        item[u] = load(cur, w0 + cur, L.start[cur] + (int64_t)(t - L.off[cur]), base + t);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -139,6 +142,54 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
     }
     base += total;
     __syncthreads();
+  }
+}
+
+// Makes this wave's LDS writes visible to its other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// block_expand for one wave working alone: windows of 64 nonzeros, wave
+// scan by shuffles, per-wave LDS prefix (w_off[64], w_start[64]); no block
+// barriers, so the waves of a block can work on different rows.
+template <int U, class Seg, class Load, class Apply>
+__device__ __forceinline__ void wave_expand(int64_t p0, int64_t p1, int32_t* w_off, int64_t* w_start, Seg seg,
+                                            Load load, Apply apply) {
+  using Item = decltype(load(0, int64_t(0), int64_t(0), int64_t(0)));
+  const int lane = lane_id();
+  int64_t base = 0;
+  for (int64_t w0 = p0; w0 < p1; w0 += kWave) {
+    const int64_t p = w0 + lane;
+    int64_t st = 0;
+    int32_t len = 0;
+    if (p < p1) seg(p, st, len);
+    const int32_t incl = wave_incl_scan(len);
+    const int32_t total = __shfl(incl, kWave - 1, kWave);
+    w_off[lane] = incl - len;
+    w_start[lane] = st;
+    wave_lds_sync();
+    const int nvalid = (int)((p1 - w0) < kWave ? (p1 - w0) : kWave);
+    int cur = 0;
+    for (int32_t t0 = 0; t0 < total; t0 += kWave * U) {
+      Item item[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        int32_t t = t0 + u * kWave + lane;
+        ok[u] = t < total;
+        t = ok[u] ? t : total - 1;
+        cur = seek_segment(w_off, nvalid, cur, t);
+        item[u] = load(cur, w0 + cur, w_start[cur] + (int64_t)(t - w_off[cur]), base + t);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ok[u]) apply(item[u]);
+    }
+    base += total;
+    wave_lds_sync();
   }
 }
 

@@ -70,6 +70,11 @@ struct SegB {
   }
 };
 
+// Products in flight per lane in the expansion (loads issued before the
+// LDS updates they feed), per kernel.
+constexpr int kUSymBitmap = 16;
+constexpr int kUDense = 8;
+
 template <typename V>
 struct KeyVal {
   int32_t key;
@@ -80,6 +85,23 @@ template <typename V>
 struct RankKeyVal {
   int32_t rank;  // product's rank in A-row order
   int32_t key;   // column
+  V val;
+};
+
+// What the expansion gathers for one product: column, window index of the
+// A nonzero, B value (the product itself is formed in apply).
+template <typename V>
+struct Gather {
+  int32_t key;
+  int32_t j;
+  V val;
+};
+
+template <typename V>
+struct RankGather {
+  int32_t rank;
+  int32_t key;
+  int32_t j;
   V val;
 };
 
@@ -112,7 +134,7 @@ __global__ __launch_bounds__(256) void k_row_flops(Csr A, Csr B, int64_t* row_fl
 // takes kFlpIpt consecutive nonzeros of A, finds its first row by binary
 // search over A.rp, and adds its partial sums to row_flp with one atomic per
 // row it touches.  row_flp must start at zero.
-constexpr int kFlpIpt = 8;
+constexpr int kFlpIpt = 64;
 __global__ __launch_bounds__(256) void k_row_flops_nz(Csr A, Csr B, unsigned long long* row_flp,
                                                       unsigned long long* total) {
   __shared__ int64_t s_w[4];
@@ -370,9 +392,10 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
-      block_expand<T, 4>(A.rp[row], A.rp[row + 1], L, seg,
-                         [&](int, int64_t, int64_t q, int64_t) { return (uint32_t)((int64_t)B.col[q] - base); },
-                         [&](uint32_t u) {
+      block_expand<T, kUSymBitmap>(A.rp[row], A.rp[row + 1], L, seg,
+                         [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+                         [&](int32_t col) {
+                           const uint32_t u = (uint32_t)((int64_t)col - base);
                            if (u < (uint32_t)pw) atomicOr(&bm[u >> 5], 1u << (u & 31));
                          });
       // words_pass rounded up to whole waves so every lane joins the vote
@@ -404,6 +427,83 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
       rownz[row] = running;
     }
     __syncthreads();
+  }
+}
+
+// Blocked symbolic for wide rows: every wave takes (row h, chunk c)
+// morsels, chunk-major, and counts the distinct columns of C's row h in
+// chunk c with a W-bit LDS bitmap of its own, reading the uint16 columns of
+// the column-blocked B.  Concurrent waves share one column block of B, which
+// stays in L2.  Output: ccnt[h * R + c].  Dynamic LDS per wave:
+//   uint32 bitmap[W/32] | int32 off[64] | int64 start[64]
+template <int T>
+__global__ __launch_bounds__(T) void k_symb_blocked(Csr A, int64_t mB, const int64_t* bptr, const uint16_t* bcol,
+                                                    const int32_t* list, const int32_t* counts, int bin, int R,
+                                                    int W, int32_t* ccnt, unsigned long long* ticket) {
+  extern __shared__ __align__(16) unsigned char dyn_raw[];
+  const int words = W >> 5;
+  const size_t per_wave = ((size_t)words * 4 + 64 * 4 + 64 * 8 + 15) & ~(size_t)15;
+  unsigned char* mine = dyn_raw + per_wave * wave_id();
+  uint32_t* bm = (uint32_t*)mine;
+  int32_t* w_off = (int32_t*)(mine + (size_t)words * 4);
+  int64_t* w_start = (int64_t*)(mine + (size_t)words * 4 + 64 * 4);
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  const unsigned long long ntask = (unsigned long long)H * (unsigned long long)R;
+  const int64_t ld = mB + 1;
+  const unsigned long long waves = (unsigned long long)gridDim.x * (T / kWave);
+  const unsigned long long per_wave_tasks = ntask / (waves * 64ull);
+  const unsigned long long grab = per_wave_tasks < 1 ? 1 : (per_wave_tasks > 16 ? 16 : per_wave_tasks);
+  for (int w = lane_id(); w < words; w += kWave) bm[w] = 0u;
+  unsigned long long tk = 0, tk_end = 0;
+  for (;;) {
+    if (tk >= tk_end) {
+      unsigned long long got = 0;
+      if (lane_id() == 0) got = atomicAdd(ticket, grab);
+      tk = __shfl(got, 0, kWave);
+      tk_end = tk + grab;
+    }
+    const unsigned long long cur = tk++;
+    if (cur >= ntask) break;
+    const int c = (int)(cur / (unsigned long long)H);
+    const int32_t h = (int32_t)(cur - (unsigned long long)c * H);
+    const int32_t row = list[start + h];
+    const int64_t* bp = bptr + (int64_t)c * ld;
+    wave_lds_sync();
+    wave_expand<kUSymBitmap>(A.rp[row], A.rp[row + 1], w_off, w_start,
+                             [&](int64_t p, int64_t& st, int32_t& len) {
+                               const int32_t k = A.col[p];
+                               st = bp[k];
+                               len = (int32_t)(bp[k + 1] - st);
+                             },
+                             [&](int, int64_t, int64_t q, int64_t) { return (uint32_t)bcol[q]; },
+                             [&](uint32_t u) { atomicOr(&bm[u >> 5], 1u << (u & 31)); });
+    int32_t pc = 0;
+    for (int w = lane_id(); w < words; w += kWave) {
+      pc += __popc(bm[w]);
+      bm[w] = 0u;
+    }
+    pc = wave_sum(pc);
+    if (lane_id() == 0) ccnt[(int64_t)h * R + c] = pc;
+  }
+}
+
+// Per wide row: coff = exclusive prefix of its chunk counts, total into
+// rownz and coff[R].
+__global__ __launch_bounds__(256) void k_coff_rows(const int32_t* list, const int32_t* counts, int bin, int R,
+                                                   const int32_t* ccnt, int32_t* coff, int64_t* rownz) {
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < H; h += (int64_t)gridDim.x * blockDim.x) {
+    int32_t run = 0;
+    const int32_t* cr = ccnt + h * R;
+    int32_t* o = coff + h * (R + 1);
+    for (int c = 0; c < R; ++c) {
+      o[c] = run;
+      run += cr[c];
+    }
+    o[R] = run;
+    rownz[list[start + h]] = run;
   }
 }
 
@@ -442,11 +542,9 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
                       len = (int32_t)(B.rp[k + 1] - st);
                       aval[threadIdx.x] = av[p];
                     },
-                    [&](int j, int64_t, int64_t q, int64_t) {
-                      return KeyVal<V>{B.col[q], aval[j] * bv[q]};
-                    },
-                    [&](KeyVal<V> kv) {
-                      if (!lds_insert_add<V>(keys, vals, shift, kv.key, kv.val)) atomicOr(err, 16);
+                    [&](int j, int64_t, int64_t q, int64_t) { return Gather<V>{B.col[q], j, bv[q]}; },
+                    [&](Gather<V> g) {
+                      if (!lds_insert_add<V>(keys, vals, shift, g.key, aval[g.j] * g.val)) atomicOr(err, 16);
                     });
     for (int s0 = 0; s0 < S; s0 += T) {
       const int s = s0 + threadIdx.x;
@@ -520,11 +618,11 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
                          aval[threadIdx.x] = av[p];
                        },
                        [&](int j, int64_t, int64_t q, int64_t t) {
-                         return RankKeyVal<V>{(int32_t)t, B.col[q], aval[j] * bv[q]};
+                         return RankGather<V>{(int32_t)t, B.col[q], j, bv[q]};
                        },
-                       [&](RankKeyVal<V> r) {
+                       [&](RankGather<V> r) {
                          key[r.rank] = (uint32_t)r.key;
-                         vals[r.rank] = r.val;
+                         vals[r.rank] = aval[r.j] * r.val;
                        });
     const int nb = (int)next_pow2_i64(f);
     if (esc_sort == 0) {
@@ -649,17 +747,72 @@ __global__ __launch_bounds__(64) void k_bblk_scatter(Csr B, Probs pr, int W, int
   }
 }
 
-// Dense-chunk numeric: persistent blocks take tickets t = c * H + h
-// (chunk-major, so concurrently running morsels read the same column block
-// of B) for heavy row h of the bitmap list and chunk c.  Dynamic LDS:
-//   V dense[W] | uint32 bitmap[W/32] | union{ExpandLds, uint16 idx[W]}
+// One morsel of the dense bin: wide row `row`, column chunk `c`.
+struct DenseTask {
+  int32_t row;
+  int32_t cnt;   // nnz of C in this chunk
+  int32_t c;
+  int32_t deg;   // nnz of A's row
+  int64_t off;   // first C slot of the chunk's entries
+  int64_t p0;    // first A nonzero of the row
+};
+
+// flag[c*H + h] = 1 when morsel (h, c) is routed dense and not empty.
+__global__ __launch_bounds__(256) void k_dense_flags(const int32_t* list, const int32_t* counts, int bin,
+                                                     const uint8_t* nbin, const int32_t* coff, int R,
+                                                     int64_t* flag) {
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  const int64_t n = (int64_t)H * R;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / H);
+    const int32_t h = (int32_t)(i - (int64_t)c * H);
+    const int32_t row = list[start + h];
+    const int32_t* crow = coff + (int64_t)h * (R + 1);
+    flag[i] = (nbin[row] == NB_DENSE && crow[c + 1] > crow[c]) ? 1 : 0;
+  }
+}
+
+// Writes the descriptors of the flagged morsels at their scanned positions
+// (pos = exclusive scan of the flags, n_all + 1 entries), keeping the
+// chunk-major order, and the morsel count into *ntask.
+__global__ __launch_bounds__(256) void k_dense_fill(Csr A, const int32_t* list, const int32_t* counts, int bin,
+                                                    const int32_t* coff, int R, const int64_t* rp_c,
+                                                    const int64_t* pos, DenseTask* tasks, int64_t* ntask) {
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  const int64_t n = (int64_t)H * R;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = pos[i];
+    if (pos[i + 1] == p) continue;  // not flagged
+    const int c = (int)(i / H);
+    const int32_t h = (int32_t)(i - (int64_t)c * H);
+    const int32_t row = list[start + h];
+    const int32_t* crow = coff + (int64_t)h * (R + 1);
+    DenseTask d;
+    d.row = row;
+    d.cnt = crow[c + 1] - crow[c];
+    d.c = c;
+    d.p0 = A.rp[row];
+    d.deg = (int32_t)(A.rp[row + 1] - d.p0);
+    d.off = rp_c[row] + crow[c];
+    tasks[p] = d;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ntask = pos[n];
+}
+
 constexpr unsigned long long kGrab = 8;
 
+// Dense-chunk numeric: persistent blocks take morsels from the compacted
+// descriptor array (chunk-major, so concurrently running morsels read the
+// same column block of B) through a global ticket counter.  A morsel is
+// accumulated in a dense LDS array and emitted by bitmap rank as one sorted,
+// coalesced run of C.  Dynamic LDS:
+//   V dense[W] | uint32 bitmap[W/32] | union{ExpandLds, uint16 idx[W]}
 template <int T, typename V>
 __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64_t* bptr,
                                                   const uint16_t* bcol, const V* bval,
-                                                  const int32_t* list, const int32_t* counts, int bin,
-                                                  const uint8_t* nbin, const int32_t* coff, int R, int W,
+                                                  const DenseTask* tasks, const int64_t* d_ntask, int W,
                                                   const int64_t* rp_c, int32_t* col_c, V* val_c,
                                                   int64_t cap, Probs pr, unsigned long long* ticket,
                                                   int32_t* err) {
@@ -673,12 +826,10 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
   __shared__ int64_t s_w[T / kWave];
   __shared__ unsigned long long s_ticket;
   if (rp_c[A.m] > cap) return;
-  int32_t start, H;
-  bin_range(counts, bin, start, H);
   const int words = W >> 5;
   for (int i = threadIdx.x; i < W; i += T) dense[i] = V(0);
   for (int i = threadIdx.x; i < words; i += T) bm[i] = 0u;
-  const unsigned long long ntask = (unsigned long long)H * (unsigned long long)R;
+  const unsigned long long ntask = (unsigned long long)*d_ntask;
   const V* av = (const V*)A.val;
   const int64_t ld = mB + 1;
   // tickets [tk, tk_end) are claimed `grab` at a time to keep the queue
@@ -687,43 +838,40 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
   const unsigned long long per_block = ntask / ((unsigned long long)gridDim.x * 64ull);
   const unsigned long long grab = per_block < 1 ? 1 : (per_block > kGrab ? kGrab : per_block);
   unsigned long long tk = 0, tk_end = 0;
+  DenseTask nxt{};
   for (;;) {
     __syncthreads();
+    bool have_next = false;
     if (tk >= tk_end) {
       if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, grab);
       __syncthreads();
       tk = s_ticket;
       tk_end = tk + grab;
+    } else {
+      have_next = true;
     }
     const unsigned long long cur = tk++;
     if (cur >= ntask) break;
-    const int c = (int)(cur / (unsigned long long)H);
-    const int32_t h = (int32_t)(cur - (unsigned long long)c * H);
-    const int32_t row = list[start + h];
-    if (nbin[row] != NB_DENSE) continue;
-    const int32_t* crow = coff + (int64_t)h * (R + 1);
-    const int32_t cbase = crow[c];
-    const int32_t ccnt = crow[c + 1] - cbase;
-    if (ccnt == 0) continue;
+    const DenseTask d = have_next ? nxt : tasks[cur];
+    if (tk < tk_end && tk < ntask) nxt = tasks[tk];  // prefetch: lands while this morsel runs
     int64_t lo, width;
-    prob_cols(pr, row, lo, width);
-    const int64_t off = rp_c[row] + cbase;
-    const int64_t* bp = bptr + (int64_t)c * ld;
-    block_expand<T, 4>(A.rp[row], A.rp[row + 1], L,
-                    [&](int64_t p, int64_t& st, int32_t& len) {
-                      const int32_t k = A.col[p];
-                      st = bp[k];
-                      len = (int32_t)(bp[k + 1] - st);
-                      aval[threadIdx.x] = av[p];
-                    },
-                    [&](int j, int64_t, int64_t q, int64_t) {
-                      return KeyVal<V>{(int32_t)bcol[q], aval[j] * bval[q]};
-                    },
-                    [&](KeyVal<V> kv) {
-                      const uint32_t u = (uint32_t)kv.key;
-                      atomicOr(&bm[u >> 5], 1u << (u & 31));
-                      atomicAdd(&dense[u], kv.val);
-                    });
+    prob_cols(pr, d.row, lo, width);
+    const int64_t* bp = bptr + (int64_t)d.c * ld;
+    block_expand<T, kUDense>(d.p0, d.p0 + d.deg, L,
+                       [&](int64_t p, int64_t& st, int32_t& len) {
+                         const int32_t k = A.col[p];
+                         st = bp[k];
+                         len = (int32_t)(bp[k + 1] - st);
+                         aval[threadIdx.x] = av[p];
+                       },
+                       [&](int j, int64_t, int64_t q, int64_t) {
+                         return Gather<V>{(int32_t)bcol[q], j, bval[q]};
+                       },
+                       [&](Gather<V> g) {
+                         const uint32_t u = (uint32_t)g.key;
+                         atomicOr(&bm[u >> 5], 1u << (u & 31));
+                         atomicAdd(&dense[u], aval[g.j] * g.val);
+                       });
     // rank the set bits: each thread owns a run of consecutive words
     const int wpt = (words + T - 1) / T;
     const int w0 = threadIdx.x * wpt;
@@ -732,7 +880,7 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
     for (int w = w0; w < w1; ++w) pc += __popc(bm[w]);
     int64_t tot;
     int32_t r = (int32_t)block_excl_scan<T, int64_t>(pc, s_w, tot);
-    if (threadIdx.x == 0 && tot != ccnt) atomicOr(err, 4);
+    if (threadIdx.x == 0 && tot != d.cnt) atomicOr(err, 4);
     for (int w = w0; w < w1; ++w) {
       uint32_t x = bm[w];
       bm[w] = 0u;
@@ -743,12 +891,14 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
       }
     }
     __syncthreads();
-    const int32_t colbase = (int32_t)(lo + (int64_t)c * W);
-    const int32_t nout = (int32_t)tot < ccnt ? (int32_t)tot : ccnt;
+    const int32_t colbase = (int32_t)(lo + (int64_t)d.c * W);
+    const int32_t nout = (int32_t)tot;
     for (int i = threadIdx.x; i < nout; i += T) {
       const uint32_t u = sel[i];
-      col_c[off + i] = colbase + (int32_t)u;
-      val_c[off + i] = dense[u];
+      if (i < d.cnt) {
+        col_c[d.off + i] = colbase + (int32_t)u;
+        val_c[d.off + i] = dense[u];
+      }
       dense[u] = V(0);
     }
   }

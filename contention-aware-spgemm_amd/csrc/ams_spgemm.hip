@@ -41,6 +41,8 @@ struct Ctrl {
   int32_t pad;
   unsigned long long flops;
   unsigned long long ticket;
+  int64_t ntask;
+  int64_t pad1;
   unsigned long long sums[16];
   int64_t pad2[5];
 };
@@ -63,7 +65,8 @@ struct ams_ctx {
   ams_stats_t stats{};
   std::string err;
   Buf row_flp, sbin, nbin, lsym, lnum, ctrl, coff, bptr, bcol, bval, bperm, tsum;
-  Buf pool_rp, pool_col, pool_val, probs, scratch_a;
+  Buf pool_rp, pool_col, pool_val, probs, scratch_a, dflag, dtask, ccnt;
+  bool bblk_ready = false;  // blocked B of the current plan already built
   Ctrl* h_ctrl = nullptr;       // pinned mirror of the control block
   int64_t* h_pin = nullptr;     // pinned scratch
   // plan of the last symbolic call
@@ -272,6 +275,31 @@ static int launch_row_flops(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t*
   return AMS_OK;
 }
 
+// Column-blocked copy of B (chunk-major CSR of width-W column blocks).
+template <typename V>
+static int build_bblk(ams_ctx_t* ctx, const Csr& B, const Probs& pr, int W, int R, bool perm) {
+  const int64_t ld = B.m + 1;
+  CHK(grow(ctx, ctx->bptr, sizeof(int64_t) * (size_t)R * ld));
+  CHK(grow(ctx, ctx->bcol, sizeof(uint16_t) * std::max<int64_t>(B.nnz, 1)));
+  CHK(grow(ctx, ctx->bval, sizeof(V) * std::max<int64_t>(B.nnz, 1)));
+  if (perm) CHK(grow(ctx, ctx->bperm, sizeof(int64_t) * std::max<int64_t>(B.nnz, 1)));
+  int64_t* bptr = (int64_t*)ctx->bptr.p;
+  CHK(fill_i64(ctx, bptr, (int64_t)R * ld, 0));
+  if (B.m > 0) {
+    const int g = (int)std::min<int64_t>(B.m, (int64_t)ctx->num_cu * 32);
+    const size_t dyn = sizeof(int32_t) * R;
+    hipLaunchKernelGGL(k_bblk_count, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R, bptr);
+    CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
+    hipLaunchKernelGGL(k_bblk_scatter<V>, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R,
+                       (const int64_t*)bptr, (uint16_t*)ctx->bcol.p, (V*)ctx->bval.p,
+                       perm ? (int64_t*)ctx->bperm.p : (int64_t*)nullptr);
+    HIPCHK(hipGetLastError());
+  } else {
+    CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
+  }
+  return AMS_OK;
+}
+
 // ------------------------------------------------------------ symbolic
 
 template <int S>
@@ -351,7 +379,35 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     launch_symb_hash<8192>(ctx, A, B, rownz, 7);
     HIPCHK(hipGetLastError());
   }
-  if (m > 0) {
+  ctx->bblk_ready = false;
+  if (m > 0 && ctx->prm.sym_blocked && (!ctx->plan_counts || H > 0)) {
+    // wide rows counted per (row, chunk) morsel on the column-blocked B
+    {
+      Phase ph(ctx, AMS_PH_BBLK, sync);
+      if (dtype == AMS_F64) CHK(build_bblk<double>(ctx, B, pr, W, R, false));
+      else CHK(build_bblk<float>(ctx, B, pr, W, R, false));
+      ctx->bblk_ready = true;
+    }
+    Phase ph(ctx, AMS_PH_SYMB_BITMAP, sync);
+    CHK(grow(ctx, ctx->ccnt, sizeof(int32_t) * std::max<int64_t>(H, 1) * R));
+    CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
+    constexpr int TB = 512;
+    const size_t per_wave = ((size_t)(W / 32) * 4 + 64 * 4 + 64 * 8 + 15) & ~(size_t)15;
+    const size_t dyn = per_wave * (TB / 64);
+    HIPCHK(hipFuncSetAttribute((const void*)k_symb_blocked<TB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dyn));
+    int occ;
+    occupancy(k_symb_blocked<TB>, TB, dyn, &occ);
+    const int g = grid_for(ctx, occ, H * R, ctx->plan_counts);
+    hipLaunchKernelGGL(k_symb_blocked<TB>, dim3(g), dim3(TB), dyn, ctx->stream, A, B.m, (const int64_t*)ctx->bptr.p,
+                       (const uint16_t*)ctx->bcol.p, (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym,
+                       (int)SB_BITMAP, R, W, (int32_t*)ctx->ccnt.p, &c->ticket);
+    const int gr = (int)std::max<int64_t>(1, std::min<int64_t>((H + 255) / 256, (int64_t)ctx->num_cu * 8));
+    hipLaunchKernelGGL(k_coff_rows, dim3(gr), dim3(256), 0, ctx->stream, (const int32_t*)ctx->lsym.p,
+                       (const int32_t*)c->cnt_sym, (int)SB_BITMAP, R, (const int32_t*)ctx->ccnt.p,
+                       (int32_t*)ctx->coff.p, rp_c);
+    HIPCHK(hipGetLastError());
+  } else if (m > 0) {
     int64_t* rownz = rp_c;
     Phase ph(ctx, AMS_PH_SYMB_BITMAP, sync);
     if (!ctx->plan_counts || H > 0) {
@@ -456,31 +512,6 @@ static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const in
                      (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
 }
 
-// Column-blocked copy of B (chunk-major CSR of width-W column blocks).
-template <typename V>
-static int build_bblk(ams_ctx_t* ctx, const Csr& B, const Probs& pr, int W, int R, bool perm) {
-  const int64_t ld = B.m + 1;
-  CHK(grow(ctx, ctx->bptr, sizeof(int64_t) * (size_t)R * ld));
-  CHK(grow(ctx, ctx->bcol, sizeof(uint16_t) * std::max<int64_t>(B.nnz, 1)));
-  CHK(grow(ctx, ctx->bval, sizeof(V) * std::max<int64_t>(B.nnz, 1)));
-  if (perm) CHK(grow(ctx, ctx->bperm, sizeof(int64_t) * std::max<int64_t>(B.nnz, 1)));
-  int64_t* bptr = (int64_t*)ctx->bptr.p;
-  CHK(fill_i64(ctx, bptr, (int64_t)R * ld, 0));
-  if (B.m > 0) {
-    const int g = (int)std::min<int64_t>(B.m, (int64_t)ctx->num_cu * 32);
-    const size_t dyn = sizeof(int32_t) * R;
-    hipLaunchKernelGGL(k_bblk_count, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R, bptr);
-    CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
-    hipLaunchKernelGGL(k_bblk_scatter<V>, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R,
-                       (const int64_t*)bptr, (uint16_t*)ctx->bcol.p, (V*)ctx->bval.p,
-                       perm ? (int64_t*)ctx->bperm.p : (int64_t*)nullptr);
-    HIPCHK(hipGetLastError());
-  } else {
-    CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
-  }
-  return AMS_OK;
-}
-
 template <typename V>
 static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, const int64_t* rp_c,
                          int32_t* col_c, V* val_c, int64_t cap) {
@@ -491,9 +522,28 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
   const bool dense = !ctx->plan_counts || ctx->cnt_num[NB_DENSE] > 0;
   const bool on = ctx->plan_counts;
   if (dense && A.m > 0) {
-    {
+    if (!ctx->bblk_ready) {
       Phase ph(ctx, AMS_PH_BBLK, on);
       CHK(build_bblk<V>(ctx, B, pr, W, R, false));
+    }
+    const int64_t hmax = ctx->plan_counts ? (int64_t)ctx->cnt_sym[SB_BITMAP] : A.m;
+    const int64_t nall = hmax * R;
+    {
+      // compacted morsel descriptors, chunk-major
+      Phase ph(ctx, AMS_PH_BBLK, on);
+      CHK(grow(ctx, ctx->dflag, sizeof(int64_t) * (nall + 1)));
+      CHK(grow(ctx, ctx->dtask, sizeof(DenseTask) * std::max<int64_t>(nall, 1)));
+      int64_t* flag = (int64_t*)ctx->dflag.p;
+      CHK(fill_i64(ctx, flag, nall + 1, 0));
+      const int gf = (int)std::max<int64_t>(1, std::min<int64_t>((nall + 255) / 256, (int64_t)ctx->num_cu * 16));
+      hipLaunchKernelGGL(k_dense_flags, dim3(gf), dim3(256), 0, ctx->stream, (const int32_t*)ctx->lsym.p,
+                         (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const uint8_t*)ctx->nbin.p,
+                         (const int32_t*)ctx->coff.p, R, flag);
+      CHK(scan_i64(ctx, flag, flag, nall + 1));
+      hipLaunchKernelGGL(k_dense_fill, dim3(gf), dim3(256), 0, ctx->stream, A, (const int32_t*)ctx->lsym.p,
+                         (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const int32_t*)ctx->coff.p, R, rp_c,
+                         (const int64_t*)flag, (DenseTask*)ctx->dtask.p, &c->ntask);
+      HIPCHK(hipGetLastError());
     }
     Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
     const size_t dyn = dense_lds(W, sizeof(V));
@@ -504,13 +554,11 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
     CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
     int occ;
     occupancy(k_numc_dense<kDenseT, V>, kDenseT, dyn, &occ);
-    const int64_t tasks = ctx->plan_counts ? (int64_t)ctx->cnt_sym[SB_BITMAP] * R : A.m;
-    const int g = grid_for(ctx, occ, tasks, ctx->plan_counts);
+    const int g = grid_for(ctx, occ, nall, ctx->plan_counts);
     hipLaunchKernelGGL((k_numc_dense<kDenseT, V>), dim3(g), dim3(kDenseT), dyn, ctx->stream, A, B.m,
                        (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
-                       (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP,
-                       (const uint8_t*)ctx->nbin.p, (const int32_t*)ctx->coff.p, R, W, rp_c, col_c, val_c, cap,
-                       pr, &c->ticket, &c->err);
+                       (const DenseTask*)ctx->dtask.p, (const int64_t*)&c->ntask, W, rp_c, col_c, val_c, cap, pr,
+                       &c->ticket, &c->err);
     HIPCHK(hipGetLastError());
   }
   if (A.m > 0) {
@@ -620,6 +668,7 @@ static int masked_bwd_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     int W, R, Wb;
     dense_geometry(ctx, sizeof(V) == 8 ? AMS_F64 : AMS_F32, pr.max_width, W, R, Wb);
     CHK(build_bblk<V>(ctx, B, pr, W, R, true));
+    ctx->bblk_ready = false;
     const size_t dyn = (size_t)W * sizeof(V);
     HIPCHK(hipFuncSetAttribute((const void*)k_bwd_dense<kDenseT, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dyn));
@@ -654,7 +703,8 @@ void ams_default_params(ams_params_t* p) {
   p->dense_density = 0.05;
   p->mode = AMS_MODE_HYBRID;
   p->wgs_per_cu = 0;
-  p->esc_sort = 1;
+  p->esc_sort = 0;
+  p->sym_blocked = 0;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -687,7 +737,7 @@ int ams_ctx_destroy(ams_ctx_t* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   Buf* bufs[] = {&ctx->row_flp, &ctx->sbin, &ctx->nbin, &ctx->lsym, &ctx->lnum, &ctx->ctrl, &ctx->coff,
                  &ctx->bptr, &ctx->bcol, &ctx->bval, &ctx->bperm, &ctx->tsum, &ctx->pool_rp, &ctx->pool_col,
-                 &ctx->pool_val, &ctx->probs, &ctx->scratch_a};
+                 &ctx->pool_val, &ctx->probs, &ctx->scratch_a, &ctx->dflag, &ctx->dtask, &ctx->ccnt};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : ctx->recs) {
@@ -721,6 +771,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (p->bitmap_bits < p->dense_width) return fail(ctx, AMS_ERR_ARG, "bitmap_bits must be >= dense_width");
   if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
   if (p->esc_sort < 0 || p->esc_sort > 1) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0 or 1");
+  if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   ctx->prm = *p;
   return AMS_OK;
 }
@@ -966,7 +1017,10 @@ int ams_batched_plan(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, i
   CHK(grow(ctx, ctx->bptr, sizeof(int64_t) * (size_t)R * ld));
   CHK(grow(ctx, ctx->bcol, sizeof(uint16_t) * std::max<int64_t>(b->num_nnz, 1)));
   CHK(grow(ctx, ctx->bval, (dtype == AMS_F64 ? 8 : 4) * std::max<int64_t>(b->num_nnz, 1)));
-  const int64_t nmax = std::max<int64_t>(m + 1, (int64_t)R * ld);
+  CHK(grow(ctx, ctx->ccnt, sizeof(int32_t) * std::max<int64_t>(m, 1) * R));
+  CHK(grow(ctx, ctx->dflag, sizeof(int64_t) * (std::max<int64_t>(m, 1) * R + 1)));
+  CHK(grow(ctx, ctx->dtask, sizeof(DenseTask) * std::max<int64_t>(m, 1) * R));
+  const int64_t nmax = std::max<int64_t>(std::max<int64_t>(m + 1, (int64_t)R * ld), std::max<int64_t>(m, 1) * R + 1);
   CHK(grow(ctx, ctx->tsum, sizeof(int64_t) * ((nmax + kScanTile - 1) / kScanTile)));
   ctx->batch_pr = pr;
   ctx->batch_m = m;
@@ -1076,6 +1130,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   for (double q : ratios) { ams_params_t p = best; p.bitmap_ratio = q; consider(p); }
   for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
   for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
+  for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.sym_blocked = e; consider(p); }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->err.clear();

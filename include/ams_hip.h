@@ -58,6 +58,7 @@ typedef struct {
 	int32_t mode;           /* AMS_MODE_*                                             */
 	int32_t wgs_per_cu;     /* persistent workgroups per CU for the queue kernels     */
 	int32_t esc_sort;       /* ESC row sort in LDS: 0 bitonic, 1 bucket               */
+	int32_t sym_blocked;    /* wide-row symbolic per (row, column chunk) on blocked B */
 } ams_params_t;
 
 /* Counters of the last symbolic / numeric call (rows per bin, tasks). */

@@ -28,6 +28,10 @@ ROUTES = {
     "bitmap_then_hash": dict(sym_hash_max=8, num_hash_max=5461, bitmap_ratio=0.0, dense_density=1.0,
                              bitmap_bits=256, dense_width=64),
     "big_tables": dict(sym_hash_max=5461, num_hash_max=5461, acc_thres=0.0),
+    "blocked_sym": dict(sym_blocked=1, sym_hash_max=64, num_hash_max=64, bitmap_ratio=0.0),
+    "blocked_sym_dense": dict(sym_blocked=1, sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=128,
+                              bitmap_bits=1024, dense_density=0.0),
+    "esc_bucket": dict(mode=2, esc_sort=1),
 }
 
 
@@ -66,7 +70,7 @@ def test_rmat_square(gpu_ctx, route, name, a, exact):
     assert_matches_oracle(c, a, a, exact)
 
 
-@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only"])
+@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only", "blocked_sym_dense", "esc_bucket"])
 def test_rectangular_unsorted(gpu_ctx, route):
     a = random_csr(70, 300, 0.05, seed=1)
     b = random_csr(300, 2000, 0.02, seed=2, sort=False)
