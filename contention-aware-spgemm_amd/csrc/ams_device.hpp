@@ -198,19 +198,17 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t key, int shift) {
 }
 
 // Key-only insert into an LDS linear-probing table of 2^(32-shift) slots.
-// Returns 1 when this call placed the key, 0 when it was present, -1 when
-// the table is full (a sizing bug: the caller raises an error flag).
+// One compare-and-swap per probe: its old value tells empty (placed), equal
+// (present) or another key (probe on).  Returns 1 when this call placed the
+// key, 0 when it was present, -1 when the table is full (a sizing bug: the
+// caller raises an error flag).
 __device__ __forceinline__ int lds_insert_key(int32_t* keys, int shift, int32_t key) {
   const uint32_t mask = (0xffffffffu >> shift);
   uint32_t h = hash_slot(key, shift);
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (k == key) return 0;
-    if (k == kEmpty) {
-      const int32_t old = atomicCAS(&keys[h], kEmpty, key);
-      if (old == kEmpty) return 1;
-      if (old == key) return 0;
-    }
+    const int32_t old = atomicCAS(&keys[h], kEmpty, key);
+    if (old == kEmpty) return 1;
+    if (old == key) return 0;
     h = (h + 1) & mask;
   }
   return -1;
@@ -223,16 +221,12 @@ __device__ __forceinline__ bool lds_insert_add(int32_t* keys, V* vals, int shift
   const uint32_t mask = (0xffffffffu >> shift);
   uint32_t h = hash_slot(key, shift);
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    const int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (k != key && k == kEmpty) {
-      const int32_t old = atomicCAS(&keys[h], kEmpty, key);
-      if (old != kEmpty && old != key) { h = (h + 1) & mask; continue; }
-    } else if (k != key) {
-      h = (h + 1) & mask;
-      continue;
+    const int32_t old = atomicCAS(&keys[h], kEmpty, key);
+    if (old == kEmpty || old == key) {
+      atomicAdd(&vals[h], v);
+      return true;
     }
-    atomicAdd(&vals[h], v);
-    return true;
+    h = (h + 1) & mask;
   }
   return false;
 }
@@ -249,18 +243,36 @@ __device__ __forceinline__ uint32_t lds_find(const int32_t* keys, int shift, int
   return mask + 1;
 }
 
-// Ascending bitonic sort of n (power of two) keys in LDS by the T threads
-// of the block.  Ends with a barrier.
-template <int T, typename K>
+// Ascending bitonic sort of n (power of two, n <= 2*T*MAXP) keys in LDS by
+// the T threads of the block.  Each thread loads all of its MAXP pairs of a
+// stage before comparing and storing any, so the LDS reads of a stage are in
+// flight together.  Ends with a barrier.
+template <int T, int MAXP, typename K>
 __device__ __forceinline__ void block_bitonic_sort(K* s, int n) {
+  const int half = n >> 1;
   for (int k = 2; k <= n; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < (n >> 1); i += T) {
-        const int a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-        const int b = a + j;
-        const K x = s[a], y = s[b];
-        const bool up = (a & k) == 0;
-        if ((x > y) == up) { s[a] = y; s[b] = x; }
+      K x[MAXP], y[MAXP];
+      int a[MAXP];
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        const int i = threadIdx.x + p * T;
+        a[p] = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+        if (i < half) {
+          x[p] = s[a[p]];
+          y[p] = s[a[p] + j];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        const int i = threadIdx.x + p * T;
+        if (i < half) {
+          const bool up = (a[p] & k) == 0;
+          if ((x[p] > y[p]) == up) {
+            s[a[p]] = y[p];
+            s[a[p] + j] = x[p];
+          }
+        }
       }
       __syncthreads();
     }

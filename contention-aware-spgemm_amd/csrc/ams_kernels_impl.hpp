@@ -561,7 +561,7 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
     const int32_t zz = s_n < z ? s_n : z;  // never sort unset keys
     for (int i = zz + threadIdx.x; i < n; i += T) sk[i] = 0x7fffffff;
     __syncthreads();
-    block_bitonic_sort<T, int32_t>(sk, n);
+    block_bitonic_sort<T, (S / 2 + T - 1) / T, int32_t>(sk, n);
     for (int r = threadIdx.x; r < z; r += T) {
       const int32_t key = sk[r];
       const uint32_t slot = lds_find(keys, shift, key);
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
         if (i < nb) sk[i] = i < f ? (((uint64_t)kr[u] << 32) | (uint64_t)i) : ~0ull;
       }
       __syncthreads();
-      block_bitonic_sort<T, uint64_t>(sk, nb);
+      block_bitonic_sort<T, (E / 2 + T - 1) / T, uint64_t>(sk, nb);
     } else {
       uint32_t kmin = 0xffffffffu, kmax = 0u;
       for (int i = threadIdx.x; i < f; i += T) {
