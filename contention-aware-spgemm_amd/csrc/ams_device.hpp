@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace ams {
 
@@ -15,6 +16,14 @@ constexpr int kEmpty = -1;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+
+// A value that is the same on every lane, moved to a scalar register (so
+// that loads addressed by it become scalar loads).
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
 
 template <typename I>
 __device__ __forceinline__ I wave_incl_scan(I x) {
@@ -122,7 +131,11 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
     __syncthreads();
     const int32_t total = (int32_t)total64;
     const int nvalid = (int)((p1 - w0) < T ? (p1 - w0) : T);
+    // the lane's current segment stays in registers: [.., seg_end) and
+    // B position = seg_bias + t; LDS is searched only when t leaves it
     int cur = 0;
+    int32_t seg_end = -1;
+    int64_t seg_bias = 0;
     for (int32_t t0 = 0; t0 < total; t0 += T * U) {
       Item item[U];
       bool ok[U];
@@ -133,8 +146,12 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
         int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
         ok[u] = t < total;
         t = ok[u] ? t : total - 1;
-        cur = seek_segment(L.off, nvalid, cur, t);
-        item[u] = load(cur, w0 + cur, L.start[cur] + (int64_t)(t - L.off[cur]), base + t);
+        if (t >= seg_end) {
+          cur = seek_segment(L.off, nvalid, cur, t);
+          seg_end = cur + 1 < nvalid ? L.off[cur + 1] : total;
+          seg_bias = L.start[cur] - (int64_t)L.off[cur];
+        }
+        item[u] = load(cur, w0 + cur, seg_bias + t, base + t);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -173,6 +190,8 @@ __device__ __forceinline__ void wave_expand(int64_t p0, int64_t p1, int32_t* w_o
     wave_lds_sync();
     const int nvalid = (int)((p1 - w0) < kWave ? (p1 - w0) : kWave);
     int cur = 0;
+    int32_t seg_end = -1;
+    int64_t seg_bias = 0;
     for (int32_t t0 = 0; t0 < total; t0 += kWave * U) {
       Item item[U];
       bool ok[U];
@@ -181,8 +200,12 @@ __device__ __forceinline__ void wave_expand(int64_t p0, int64_t p1, int32_t* w_o
         int32_t t = t0 + u * kWave + lane;
         ok[u] = t < total;
         t = ok[u] ? t : total - 1;
-        cur = seek_segment(w_off, nvalid, cur, t);
-        item[u] = load(cur, w0 + cur, w_start[cur] + (int64_t)(t - w_off[cur]), base + t);
+        if (t >= seg_end) {
+          cur = seek_segment(w_off, nvalid, cur, t);
+          seg_end = cur + 1 < nvalid ? w_off[cur + 1] : total;
+          seg_bias = w_start[cur] - (int64_t)w_off[cur];
+        }
+        item[u] = load(cur, w0 + cur, seg_bias + t, base + t);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)

@@ -803,75 +803,210 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, const int32_t* list, 
 
 constexpr unsigned long long kGrab = 8;
 
-// Dense-chunk numeric: persistent blocks take morsels from the compacted
-// descriptor array (chunk-major, so concurrently running morsels read the
-// same column block of B) through a global ticket counter.  A morsel is
-// accumulated in a dense LDS array and emitted by bitmap rank as one sorted,
-// coalesced run of C.  Dynamic LDS:
-//   V dense[W] | uint32 bitmap[W/32] | union{ExpandLds, uint16 idx[W]}
+// A non-empty segment of a dense window: product t of the window (t in
+// [off, off + len)) reads B position bias + t and is scaled by a.
+template <typename V>
+struct __align__(16) SegBias {
+  int64_t bias;
+  V a;
+};
+
+template <typename V>
+struct DenseItem {
+  uint32_t key;  // column inside the chunk
+  V val;         // B value
+  V a;           // A value
+};
+
+// Dense-chunk numeric: persistent blocks take morsels (wide row, column
+// chunk) from the compacted descriptor array through a global ticket
+// counter; the array is chunk-major, so concurrently running morsels read
+// the same column block of B.  A morsel is accumulated in a dense LDS array
+// and emitted by bitmap rank as one sorted, coalesced run of C.
+//
+// Per window of T nonzeros of the A row, products are numbered by an
+// exclusive scan of the segment lengths and handed out in passes of up to
+// T*U: wave w takes pass positions [w*64*ue, (w+1)*64*ue) with ue =
+// ceil(n/T), lane l every 64th of them.  A pass locates the segment of each
+// position without searching: the non-empty segments are numbered
+// (compact index), each one starting inside the pass sets its start bit in
+// a pass bitmap, and the segment of position x is
+//   cover + (bits set at positions <= x),
+// with cover the segment holding the pass's first product.  The bitmaps are
+// double buffered (each pass clears the other one), so a pass costs two
+// barriers.  The next morsel's descriptor, A columns and B segment bounds
+// are loaded while the current one runs.
+//
+// Dynamic LDS: V dense[W] | uint32 bitmap[W/32] (padded to 16 bytes) |
+//              union{SegBias<V> seg[T], uint16 idx[W]}
 template <int T, typename V>
-__global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64_t* bptr,
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_numc_dense(
+                                                  Csr A, int64_t mB, const int64_t* bptr,
                                                   const uint16_t* bcol, const V* bval,
-                                                  const DenseTask* tasks, const int64_t* d_ntask, int W,
+                                                  const DenseTask* __restrict__ tasks,
+                                                  const int64_t* __restrict__ d_ntask, int W,
                                                   const int64_t* rp_c, int32_t* col_c, V* val_c,
                                                   int64_t cap, Probs pr, unsigned long long* ticket,
                                                   int32_t* err) {
+  constexpr int U = kUDense;
+  constexpr int kPassWords = T * U / 32;
+  static_assert(kPassWords <= 2 * kWave, "one wave scans the pass bitmap");
   extern __shared__ __align__(16) unsigned char dyn_raw[];
   V* dense = (V*)dyn_raw;
   uint32_t* bm = (uint32_t*)(dense + W);
-  unsigned char* uni = (unsigned char*)(bm + (W >> 5));
-  ExpandLds<T>& L = *(ExpandLds<T>*)uni;
+  unsigned char* uni = dyn_raw + (((size_t)W * sizeof(V) + (size_t)(W >> 5) * 4 + 15) & ~(size_t)15);
+  SegBias<V>* sb = (SegBias<V>*)uni;
   uint16_t* sel = (uint16_t*)uni;
-  __shared__ V aval[T];
+  __shared__ uint32_t pbits[2][kPassWords];
+  __shared__ int32_t wpre[kPassWords];
+  __shared__ int32_t cover[2];
   __shared__ int64_t s_w[T / kWave];
-  __shared__ unsigned long long s_ticket;
+  __shared__ unsigned long long s_ring[4];
   if (rp_c[A.m] > cap) return;
   const int words = W >> 5;
   for (int i = threadIdx.x; i < W; i += T) dense[i] = V(0);
   for (int i = threadIdx.x; i < words; i += T) bm[i] = 0u;
+  for (int i = threadIdx.x; i < 2 * kPassWords; i += T) (&pbits[0][0])[i] = 0u;
   const unsigned long long ntask = (unsigned long long)*d_ntask;
   const V* av = (const V*)A.val;
   const int64_t ld = mB + 1;
-  // tickets [tk, tk_end) are claimed `grab` at a time to keep the queue
-  // counter cold: up to kGrab when there are many morsels per block, one at
-  // a time when morsels are few (and then often very uneven)
+  // Ticket stream of this block, kept by thread 0: batches of `grab`
+  // tickets (up to kGrab when morsels are many, one at a time when few and
+  // uneven), the next batch claimed one batch ahead.  s_ring[i & 3] is the
+  // ticket of the block's i-th morsel, published two morsels ahead.
   const unsigned long long per_block = ntask / ((unsigned long long)gridDim.x * 64ull);
   const unsigned long long grab = per_block < 1 ? 1 : (per_block > kGrab ? kGrab : per_block);
-  unsigned long long tk = 0, tk_end = 0;
-  DenseTask nxt{};
-  for (;;) {
-    __syncthreads();
-    bool have_next = false;
-    if (tk >= tk_end) {
-      if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, grab);
-      __syncthreads();
-      tk = s_ticket;
-      tk_end = tk + grab;
-    } else {
-      have_next = true;
+  unsigned long long q_tk = 0, q_end = 0, q_nb = 0;
+  auto take = [&]() -> unsigned long long {
+    if (q_tk == q_end) {
+      q_tk = q_nb;
+      q_end = q_nb + grab;
+      q_nb = atomicAdd(ticket, grab);
     }
-    const unsigned long long cur = tk++;
-    if (cur >= ntask) break;
-    const DenseTask d = have_next ? nxt : tasks[cur];
-    if (tk < tk_end && tk < ntask) nxt = tasks[tk];  // prefetch: lands while this morsel runs
+    return q_tk++;
+  };
+  if (threadIdx.x == 0) {
+    q_nb = atomicAdd(ticket, grab);
+    s_ring[0] = take();
+    s_ring[1] = take();
+    s_ring[2] = take();
+  }
+  __syncthreads();
+  unsigned long long t_cur = uniform_u64(s_ring[0]), t_nxt = uniform_u64(s_ring[1]);
+  DenseTask d{}, nd{};
+  if (t_cur < ntask) d = tasks[t_cur];
+  if (t_nxt < ntask) nd = tasks[t_nxt];
+  // this thread's segment of the current morsel's first window
+  int64_t st = 0;
+  int32_t len = 0;
+  V a = V(0);
+  if (t_cur < ntask && (int)threadIdx.x < d.deg) {
+    const int32_t k = A.col[d.p0 + threadIdx.x];
+    a = av[d.p0 + threadIdx.x];
+    const int64_t* bp = bptr + (int64_t)d.c * ld;
+    st = bp[k];
+    len = (int32_t)(bp[k + 1] - st);
+  }
+  unsigned ps = 0;  // pass counter: parity picks the pass bitmap
+  for (unsigned mc = 0; t_cur < ntask; ++mc) {
+    // slot (mc+3)&3 was last read at morsel mc-1 and is read next at mc+1;
+    // the barriers of every morsel order both
+    const unsigned long long t_nn = uniform_u64(s_ring[(mc + 2) & 3]);
+    if (threadIdx.x == 0) s_ring[(mc + 3) & 3] = take();
+    // prefetch: descriptor two morsels ahead, A row of the next morsel
+    DenseTask nnd{};
+    if (t_nn < ntask) nnd = tasks[t_nn];
+    const bool npre = t_nxt < ntask && (int)threadIdx.x < nd.deg;
+    int32_t nk = 0;
+    V na = V(0);
+    if (npre) {
+      nk = A.col[nd.p0 + threadIdx.x];
+      na = av[nd.p0 + threadIdx.x];
+    }
     int64_t lo, width;
     prob_cols(pr, d.row, lo, width);
     const int64_t* bp = bptr + (int64_t)d.c * ld;
-    block_expand<T, kUDense>(d.p0, d.p0 + d.deg, L,
-                       [&](int64_t p, int64_t& st, int32_t& len) {
-                         const int32_t k = A.col[p];
-                         st = bp[k];
-                         len = (int32_t)(bp[k + 1] - st);
-                         aval[threadIdx.x] = av[p];
-                       },
-                       [&](int j, int64_t, int64_t q, int64_t) {
-                         return Gather<V>{(int32_t)bcol[q], j, bval[q]};
-                       },
-                       [&](Gather<V> g) {
-                         const uint32_t u = (uint32_t)g.key;
-                         atomicOr(&bm[u >> 5], 1u << (u & 31));
-                         atomicAdd(&dense[u], aval[g.j] * g.val);
-                       });
+    const int64_t p1 = d.p0 + d.deg;
+    for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
+      if (w0 != d.p0) {  // later windows of long A rows: loaded here
+        st = 0;
+        len = 0;
+        a = V(0);
+        if (w0 + threadIdx.x < p1) {
+          const int32_t k = A.col[w0 + threadIdx.x];
+          a = av[w0 + threadIdx.x];
+          st = bp[k];
+          len = (int32_t)(bp[k + 1] - st);
+        }
+      }
+      // scan of (non-empty count << 32 | length): offsets and compact index
+      int64_t tot64;
+      const int64_t ex = block_excl_scan<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
+      const int32_t off = (int32_t)(ex & 0xffffffffll);
+      const int32_t ci = (int32_t)(ex >> 32);
+      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
+      if (len > 0) sb[ci] = SegBias<V>{st - off, a};
+      for (int32_t t0 = 0; t0 < total; t0 += T * U, ++ps) {
+        const int P = ps & 1;
+        const int32_t n = (total - t0) < T * U ? (total - t0) : T * U;
+        if (len > 0) {
+          if (off <= t0 && t0 < off + len) {
+            cover[P] = ci;
+          } else if (off > t0 && off < t0 + n) {
+            const int32_t x = off - t0;
+            atomicOr(&pbits[P][x >> 5], 1u << (x & 31));
+          }
+        }
+        __syncthreads();
+        if (wave_id() == 0) {
+          const int w = 2 * lane_id();
+          const int c0 = w < kPassWords ? __popc(pbits[P][w]) : 0;
+          const int c1 = w + 1 < kPassWords ? __popc(pbits[P][w + 1]) : 0;
+          const int incl = wave_incl_scan(c0 + c1);
+          if (w < kPassWords) wpre[w] = incl - c0 - c1;
+          if (w + 1 < kPassWords) wpre[w + 1] = incl - c1;
+        }
+        // the other bitmap's last readers finished before the barrier above
+        for (int i = threadIdx.x; i < kPassWords; i += T) pbits[P ^ 1][i] = 0u;
+        __syncthreads();
+        const int ue = (n + T - 1) / T;
+        const int xb = wave_id() * kWave * ue + lane_id();
+        const int cov = cover[P];
+        DenseItem<V> item[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          ok[u] = false;
+          if (u < ue) {
+            int x = xb + u * kWave;
+            ok[u] = x < n;
+            x = ok[u] ? x : n - 1;
+            const uint32_t wb = pbits[P][x >> 5];
+            const int c = cov + wpre[x >> 5] + __popc(wb & ((2u << (x & 31)) - 1u));
+            const SegBias<V> s = sb[c];
+            const int64_t q = s.bias + t0 + x;
+            item[u] = DenseItem<V>{(uint32_t)bcol[q], bval[q], s.a};
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (ok[u]) {
+            const uint32_t k = item[u].key;
+            atomicOr(&bm[k >> 5], 1u << (k & 31));
+            atomicAdd(&dense[k], item[u].a * item[u].val);
+          }
+        }
+      }
+    }
+    // the next morsel's B segment bounds (its A columns have arrived)
+    int64_t nst = 0;
+    int32_t nlen = 0;
+    if (npre) {
+      const int64_t* nbp = bptr + (int64_t)nd.c * ld;
+      nst = nbp[nk];
+      nlen = (int32_t)(nbp[nk + 1] - nst);
+    }
+    __syncthreads();  // every product of the morsel is in dense/bm
     // rank the set bits: each thread owns a run of consecutive words
     const int wpt = (words + T - 1) / T;
     const int w0 = threadIdx.x * wpt;
@@ -901,6 +1036,13 @@ __global__ __launch_bounds__(T) void k_numc_dense(Csr A, int64_t mB, const int64
       }
       dense[u] = V(0);
     }
+    d = nd;
+    nd = nnd;
+    t_cur = t_nxt;
+    t_nxt = t_nn;
+    st = nst;
+    len = nlen;
+    a = na;
   }
 }
 

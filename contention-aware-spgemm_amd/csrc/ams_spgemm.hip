@@ -251,10 +251,16 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
   (void)dtype;
 }
 
+// Dynamic LDS of k_numc_dense: dense values, column bitmap (padded to 16
+// bytes), then the segment table of a window or the ranked column list.
 static size_t dense_lds(int W, int vbytes) {
-  const size_t uni = std::max(sizeof(ExpandLds<kDenseT>), (size_t)W * 2);
-  return (size_t)W * vbytes + (size_t)(W / 32) * 4 + ((uni + 15) & ~(size_t)15);
+  const size_t head = ((size_t)W * vbytes + (size_t)(W / 32) * 4 + 15) & ~(size_t)15;
+  const size_t uni = std::max(sizeof(SegBias<double>) * kDenseT, (size_t)W * 2);
+  return head + ((uni + 15) & ~(size_t)15);
 }
+// Static LDS of k_numc_dense (pass bitmaps, prefix counts, scan and ticket
+// scratch), bounded from above.
+constexpr size_t kDenseStaticLds = (3 * kDenseT * kUDense / 32 + 2) * 4 + 8 * (kDenseT / 64) + 64;
 
 // Row flops: a wave per row, or the nonzero split when rows are long on
 // average (a wave per row would leave most of the GPU idle).
@@ -547,7 +553,7 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
     }
     Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
     const size_t dyn = dense_lds(W, sizeof(V));
-    if (dyn + sizeof(V) * kDenseT + 8 * (kDenseT / 64) + 16 > ctx->lds_max)
+    if (dyn + kDenseStaticLds > ctx->lds_max)
       return fail(ctx, AMS_ERR_ARG, "dense_width %d needs %zu B of LDS", W, dyn);
     HIPCHK(hipFuncSetAttribute((const void*)k_numc_dense<kDenseT, V>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
