@@ -73,7 +73,12 @@ struct SegB {
 // Products in flight per lane in the expansion (loads issued before the
 // LDS updates they feed), per kernel.
 constexpr int kUSymBitmap = 16;
-constexpr int kUDense = 8;
+// products per thread and pass of the dense numeric kernel (register bound)
+#ifndef AMS_DENSE_U1024
+#define AMS_DENSE_U1024 6
+#endif
+template <int T>
+constexpr int dense_u() { return T >= 1024 ? AMS_DENSE_U1024 : 8; }
 
 template <typename V>
 struct KeyVal {
@@ -395,8 +400,13 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
       block_expand<T, kUSymBitmap>(A.rp[row], A.rp[row + 1], L, seg,
                          [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
                          [&](int32_t col) {
+                           // hub columns recur: test the bit first, so that only
+                           // unset bits take the (same-word serialised) atomic
                            const uint32_t u = (uint32_t)((int64_t)col - base);
-                           if (u < (uint32_t)pw) atomicOr(&bm[u >> 5], 1u << (u & 31));
+                           if (u < (uint32_t)pw) {
+                             const uint32_t bit = 1u << (u & 31);
+                             if (!(bm[u >> 5] & bit)) atomicOr(&bm[u >> 5], bit);
+                           }
                          });
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {
@@ -839,8 +849,11 @@ struct DenseItem {
 //
 // Dynamic LDS: V dense[W] | uint32 bitmap[W/32] (padded to 16 bytes) |
 //              union{SegBias<V> seg[T], uint16 idx[W]}
+#ifndef AMS_DENSE_WPE
+#define AMS_DENSE_WPE 4
+#endif
 template <int T, typename V>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_numc_dense(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? AMS_DENSE_WPE : 4))) void k_numc_dense(
                                                   Csr A, int64_t mB, const int64_t* bptr,
                                                   const uint16_t* bcol, const V* bval,
                                                   const DenseTask* __restrict__ tasks,
@@ -848,9 +861,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_n
                                                   const int64_t* rp_c, int32_t* col_c, V* val_c,
                                                   int64_t cap, Probs pr, unsigned long long* ticket,
                                                   int32_t* err) {
-  constexpr int U = kUDense;
+  constexpr int U = dense_u<T>();
   constexpr int kPassWords = T * U / 32;
-  static_assert(kPassWords <= 2 * kWave, "one wave scans the pass bitmap");
+  constexpr int kWpl = kPassWords / kWave;  // pass bitmap words per lane of wave 0
+  static_assert(kPassWords % kWave == 0, "one wave scans the pass bitmap");
   extern __shared__ __align__(16) unsigned char dyn_raw[];
   V* dense = (V*)dyn_raw;
   uint32_t* bm = (uint32_t*)(dense + W);
@@ -913,31 +927,32 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_n
     // the barriers of every morsel order both
     const unsigned long long t_nn = uniform_u64(s_ring[(mc + 2) & 3]);
     if (threadIdx.x == 0) s_ring[(mc + 3) & 3] = take();
-    // prefetch: descriptor two morsels ahead, A row of the next morsel
-    DenseTask nnd{};
+    DenseTask nnd{};  // descriptor two morsels ahead
     if (t_nn < ntask) nnd = tasks[t_nn];
-    const bool npre = t_nxt < ntask && (int)threadIdx.x < nd.deg;
-    int32_t nk = 0;
-    V na = V(0);
-    if (npre) {
-      nk = A.col[nd.p0 + threadIdx.x];
-      na = av[nd.p0 + threadIdx.x];
-    }
     int64_t lo, width;
     prob_cols(pr, d.row, lo, width);
-    const int64_t* bp = bptr + (int64_t)d.c * ld;
     const int64_t p1 = d.p0 + d.deg;
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
-      if (w0 != d.p0) {  // later windows of long A rows: loaded here
-        st = 0;
-        len = 0;
-        a = V(0);
-        if (w0 + threadIdx.x < p1) {
-          const int32_t k = A.col[w0 + threadIdx.x];
-          a = av[w0 + threadIdx.x];
-          st = bp[k];
-          len = (int32_t)(bp[k + 1] - st);
-        }
+      // prefetch the A entries of the next window: this row's next one, or
+      // the next morsel's first; their B bounds follow after the passes
+      const bool last = w0 + T >= p1;
+      int64_t np = 0;
+      int32_t nn = 0, ncw = 0;
+      if (!last) {
+        np = w0 + T;
+        nn = (int32_t)((p1 - np) < T ? (p1 - np) : T);
+        ncw = d.c;
+      } else if (t_nxt < ntask) {
+        np = nd.p0;
+        nn = nd.deg < T ? nd.deg : T;
+        ncw = nd.c;
+      }
+      const bool npre = (int)threadIdx.x < nn;
+      int32_t nk = 0;
+      V na = V(0);
+      if (npre) {
+        nk = A.col[np + threadIdx.x];
+        na = av[np + threadIdx.x];
       }
       // scan of (non-empty count << 32 | length): offsets and compact index
       int64_t tot64;
@@ -958,13 +973,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_n
           }
         }
         __syncthreads();
-        if (wave_id() == 0) {
-          const int w = 2 * lane_id();
-          const int c0 = w < kPassWords ? __popc(pbits[P][w]) : 0;
-          const int c1 = w + 1 < kPassWords ? __popc(pbits[P][w + 1]) : 0;
-          const int incl = wave_incl_scan(c0 + c1);
-          if (w < kPassWords) wpre[w] = incl - c0 - c1;
-          if (w + 1 < kPassWords) wpre[w + 1] = incl - c1;
+        if (wave_id() == 0) {  // prefix popcounts: lane l owns words [l*WPL, l*WPL + WPL)
+          int cw[kWpl];
+          int sum = 0;
+#pragma unroll
+          for (int q = 0; q < kWpl; ++q) {
+            cw[q] = __popc(pbits[P][lane_id() * kWpl + q]);
+            sum += cw[q];
+          }
+          int run = wave_incl_scan(sum) - sum;
+#pragma unroll
+          for (int q = 0; q < kWpl; ++q) {
+            wpre[lane_id() * kWpl + q] = run;
+            run += cw[q];
+          }
         }
         // the other bitmap's last readers finished before the barrier above
         for (int i = threadIdx.x; i < kPassWords; i += T) pbits[P ^ 1][i] = 0u;
@@ -997,14 +1019,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_n
           }
         }
       }
-    }
-    // the next morsel's B segment bounds (its A columns have arrived)
-    int64_t nst = 0;
-    int32_t nlen = 0;
-    if (npre) {
-      const int64_t* nbp = bptr + (int64_t)nd.c * ld;
-      nst = nbp[nk];
-      nlen = (int32_t)(nbp[nk + 1] - nst);
+      // the next window's B segment bounds (its A columns have arrived)
+      st = 0;
+      len = 0;
+      a = na;
+      if (npre) {
+        const int64_t* nbp = bptr + (int64_t)ncw * ld;
+        st = nbp[nk];
+        len = (int32_t)(nbp[nk + 1] - st);
+      }
     }
     __syncthreads();  // every product of the morsel is in dense/bm
     // rank the set bits: each thread owns a run of consecutive words
@@ -1040,9 +1063,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_n
     nd = nnd;
     t_cur = t_nxt;
     t_nxt = t_nn;
-    st = nst;
-    len = nlen;
-    a = na;
   }
 }
 

@@ -50,6 +50,7 @@ static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 
 constexpr int kSymBitmapT = 512;
 constexpr int kDenseT = 512;
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 
 template <int S>
 constexpr int hash_threads() { return S / 8 < 64 ? 64 : (S / 8 > 512 ? 512 : S / 8); }
@@ -251,16 +252,27 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
   (void)dtype;
 }
 
-// Dynamic LDS of k_numc_dense: dense values, column bitmap (padded to 16
-// bytes), then the segment table of a window or the ranked column list.
+// Dynamic LDS of k_numc_dense<T>: dense values, column bitmap (padded to
+// 16 bytes), then the segment table of a window or the ranked column list.
+template <int T>
 static size_t dense_lds(int W, int vbytes) {
   const size_t head = ((size_t)W * vbytes + (size_t)(W / 32) * 4 + 15) & ~(size_t)15;
-  const size_t uni = std::max(sizeof(SegBias<double>) * kDenseT, (size_t)W * 2);
+  const size_t uni = std::max(sizeof(SegBias<double>) * T, (size_t)W * 2);
   return head + ((uni + 15) & ~(size_t)15);
 }
-// Static LDS of k_numc_dense (pass bitmaps, prefix counts, scan and ticket
-// scratch), bounded from above.
-constexpr size_t kDenseStaticLds = (3 * kDenseT * kUDense / 32 + 2) * 4 + 8 * (kDenseT / 64) + 64;
+// Static LDS of k_numc_dense<T> (pass bitmaps, prefix counts, scan and
+// ticket scratch), bounded from above.
+template <int T>
+constexpr size_t dense_static_lds() { return (3 * T * dense_u<T>() / 32 + 2) * 4 + 8 * (T / 64) + 64; }
+
+// Dense numeric block: 512 threads while two blocks fit a CU's LDS, else
+// 1024 threads so that one block still brings 16 waves.
+static int dense_block(int W, int vbytes) {
+#ifdef AMS_DENSE_FORCE_T
+  return AMS_DENSE_FORCE_T;
+#endif
+  return dense_lds<512>(W, vbytes) + dense_static_lds<512>() <= (size_t)kLdsPerCu / 2 ? 512 : 1024;
+}
 
 // Row flops: a wave per row, or the nonzero split when rows are long on
 // average (a wave per row would leave most of the GPU idle).
@@ -518,6 +530,25 @@ static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const in
                      (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
 }
 
+template <int T, typename V>
+static int launch_numc_dense(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, int64_t nall, const int64_t* rp_c,
+                             int32_t* col_c, V* val_c, int64_t cap, const Probs& pr) {
+  Ctrl* c = dctrl(ctx);
+  const size_t dyn = dense_lds<T>(W, sizeof(V));
+  if (dyn + dense_static_lds<T>() > ctx->lds_max)
+    return fail(ctx, AMS_ERR_ARG, "dense_width %d needs %zu B of LDS", W, dyn + dense_static_lds<T>());
+  HIPCHK(hipFuncSetAttribute((const void*)k_numc_dense<T, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)dyn));
+  int occ;
+  occupancy(k_numc_dense<T, V>, T, dyn, &occ);
+  const int g = grid_for(ctx, occ, nall, ctx->plan_counts);
+  hipLaunchKernelGGL((k_numc_dense<T, V>), dim3(g), dim3(T), dyn, ctx->stream, A, B.m, (const int64_t*)ctx->bptr.p,
+                     (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p, (const DenseTask*)ctx->dtask.p,
+                     (const int64_t*)&c->ntask, W, rp_c, col_c, val_c, cap, pr, &c->ticket, &c->err);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
 template <typename V>
 static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, const int64_t* rp_c,
                          int32_t* col_c, V* val_c, int64_t cap) {
@@ -552,20 +583,11 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
       HIPCHK(hipGetLastError());
     }
     Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
-    const size_t dyn = dense_lds(W, sizeof(V));
-    if (dyn + kDenseStaticLds > ctx->lds_max)
-      return fail(ctx, AMS_ERR_ARG, "dense_width %d needs %zu B of LDS", W, dyn);
-    HIPCHK(hipFuncSetAttribute((const void*)k_numc_dense<kDenseT, V>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
-    int occ;
-    occupancy(k_numc_dense<kDenseT, V>, kDenseT, dyn, &occ);
-    const int g = grid_for(ctx, occ, nall, ctx->plan_counts);
-    hipLaunchKernelGGL((k_numc_dense<kDenseT, V>), dim3(g), dim3(kDenseT), dyn, ctx->stream, A, B.m,
-                       (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
-                       (const DenseTask*)ctx->dtask.p, (const int64_t*)&c->ntask, W, rp_c, col_c, val_c, cap, pr,
-                       &c->ticket, &c->err);
-    HIPCHK(hipGetLastError());
+    const int rc = dense_block(W, sizeof(V)) == 512
+                       ? launch_numc_dense<512, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr)
+                       : launch_numc_dense<1024, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr);
+    CHK(rc);
   }
   if (A.m > 0) {
     Phase ph(ctx, AMS_PH_NUMC_HASH, on);
