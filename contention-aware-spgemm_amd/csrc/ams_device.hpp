@@ -25,22 +25,64 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// DPP lane move of a wave64 (GFX9 controls): row_shr:n reads lane i - n of
+// the same 16-lane row, row_bcast:15 / row_bcast:31 read the last lane of
+// the previous row / of row 1 into the rows of kRowMask; lanes without a
+// source (or outside kRowMask) get 0.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, kRowMask, 0xf, false);
+}
+
+template <int kCtrl, int kRowMask, typename I>
+__device__ __forceinline__ I dpp_move_int(I v) {
+  if constexpr (sizeof(I) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = dpp_move<kCtrl, kRowMask>((uint32_t)u);
+    const uint32_t hi = dpp_move<kCtrl, kRowMask>((uint32_t)(u >> 32));
+    return (I)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (I)dpp_move<kCtrl, kRowMask>((uint32_t)v);
+  }
+}
+
+// Inclusive prefix sum over the 64 lanes (integers): Hillis-Steele inside
+// each row of 16 lanes, then the row totals carried forward by broadcasts.
+// No LDS traffic.
 template <typename I>
 __device__ __forceinline__ I wave_incl_scan(I x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    I y = __shfl_up(x, d, kWave);
-    if (lane >= d) x += y;
-  }
+  static_assert(std::is_integral<I>::value && (sizeof(I) == 4 || sizeof(I) == 8), "32/64-bit integers");
+  x += dpp_move_int<0x111, 0xf>(x);  // row_shr:1
+  x += dpp_move_int<0x112, 0xf>(x);  // row_shr:2
+  x += dpp_move_int<0x114, 0xf>(x);  // row_shr:4
+  x += dpp_move_int<0x118, 0xf>(x);  // row_shr:8
+  x += dpp_move_int<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_move_int<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
   return x;
+}
+
+// Value of lane 63, in a scalar register.
+template <typename I>
+__device__ __forceinline__ I wave_last(I x) {
+  if constexpr (sizeof(I) == 8) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, kWave - 1);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), kWave - 1);
+    return (I)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (I)__builtin_amdgcn_readlane((int)x, kWave - 1);
+  }
 }
 
 template <typename I>
 __device__ __forceinline__ I wave_sum(I x) {
+  if constexpr (std::is_integral<I>::value) {
+    return wave_last(wave_incl_scan(x));
+  } else {
 #pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-  return x;
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+  }
 }
 
 // Exclusive scan over the T threads of a block.  s_wave holds T/64 values.
@@ -184,7 +226,7 @@ __device__ __forceinline__ void wave_expand(int64_t p0, int64_t p1, int32_t* w_o
     int32_t len = 0;
     if (p < p1) seg(p, st, len);
     const int32_t incl = wave_incl_scan(len);
-    const int32_t total = __shfl(incl, kWave - 1, kWave);
+    const int32_t total = wave_last(incl);
     w_off[lane] = incl - len;
     w_start[lane] = st;
     wave_lds_sync();

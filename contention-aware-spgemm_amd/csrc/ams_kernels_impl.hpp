@@ -813,6 +813,28 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, const int32_t* list, 
 
 constexpr unsigned long long kGrab = 8;
 
+// A morsel descriptor held as one dword per lane (lanes 0..7 of every
+// wave): it stays a vector load, so waits on LDS results never also wait
+// for it, and its fields are read out only when the morsel starts.
+__device__ __forceinline__ uint32_t task_load(const DenseTask* tasks, unsigned long long t,
+                                              unsigned long long ntask) {
+  uint32_t v = 0;
+  if (t < ntask && lane_id() < (int)(sizeof(DenseTask) / 4)) v = ((const uint32_t*)(tasks + t))[lane_id()];
+  return v;
+}
+
+__device__ __forceinline__ DenseTask task_get(uint32_t v) {
+  auto lane = [&](int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); };
+  DenseTask d;
+  d.row = (int32_t)lane(0);
+  d.cnt = (int32_t)lane(1);
+  d.c = (int32_t)lane(2);
+  d.deg = (int32_t)lane(3);
+  d.off = (int64_t)(((uint64_t)lane(5) << 32) | lane(4));
+  d.p0 = (int64_t)(((uint64_t)lane(7) << 32) | lane(6));
+  return d;
+}
+
 // A non-empty segment of a dense window: product t of the window (t in
 // [off, off + len)) reads B position bias + t and is scaled by a.
 template <typename V>
@@ -848,7 +870,7 @@ struct DenseItem {
 // are loaded while the current one runs.
 //
 // Dynamic LDS: V dense[W] | uint32 bitmap[W/32] (padded to 16 bytes) |
-//              union{SegBias<V> seg[T], uint16 idx[W]}
+//              SegBias<V> seg[T]
 #ifndef AMS_DENSE_WPE
 #define AMS_DENSE_WPE 4
 #endif
@@ -856,7 +878,7 @@ template <int T, typename V>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? AMS_DENSE_WPE : 4))) void k_numc_dense(
                                                   Csr A, int64_t mB, const int64_t* bptr,
                                                   const uint16_t* bcol, const V* bval,
-                                                  const DenseTask* __restrict__ tasks,
+                                                  const DenseTask* tasks,
                                                   const int64_t* __restrict__ d_ntask, int W,
                                                   const int64_t* rp_c, int32_t* col_c, V* val_c,
                                                   int64_t cap, Probs pr, unsigned long long* ticket,
@@ -870,11 +892,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   uint32_t* bm = (uint32_t*)(dense + W);
   unsigned char* uni = dyn_raw + (((size_t)W * sizeof(V) + (size_t)(W >> 5) * 4 + 15) & ~(size_t)15);
   SegBias<V>* sb = (SegBias<V>*)uni;
-  uint16_t* sel = (uint16_t*)uni;
   __shared__ uint32_t pbits[2][kPassWords];
-  __shared__ int32_t wpre[kPassWords];
+  __shared__ uint2 pword[kPassWords];  // per pass word: (start bits, set bits before the word)
   __shared__ int32_t cover[2];
   __shared__ int64_t s_w[T / kWave];
+  __shared__ int32_t s_rank[T / kWave];
   __shared__ unsigned long long s_ring[4];
   if (rp_c[A.m] > cap) return;
   const int words = W >> 5;
@@ -907,9 +929,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   }
   __syncthreads();
   unsigned long long t_cur = uniform_u64(s_ring[0]), t_nxt = uniform_u64(s_ring[1]);
-  DenseTask d{}, nd{};
-  if (t_cur < ntask) d = tasks[t_cur];
-  if (t_nxt < ntask) nd = tasks[t_nxt];
+  uint32_t dv = task_load(tasks, t_cur, ntask), ndv = task_load(tasks, t_nxt, ntask);
+  DenseTask d = task_get(dv);
   // this thread's segment of the current morsel's first window
   int64_t st = 0;
   int32_t len = 0;
@@ -927,8 +948,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     // the barriers of every morsel order both
     const unsigned long long t_nn = uniform_u64(s_ring[(mc + 2) & 3]);
     if (threadIdx.x == 0) s_ring[(mc + 3) & 3] = take();
-    DenseTask nnd{};  // descriptor two morsels ahead
-    if (t_nn < ntask) nnd = tasks[t_nn];
+    const uint32_t nndv = task_load(tasks, t_nn, ntask);  // descriptor two morsels ahead
+    const DenseTask nd = task_get(ndv);
     int64_t lo, width;
     prob_cols(pr, d.row, lo, width);
     const int64_t p1 = d.p0 + d.deg;
@@ -984,7 +1005,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
           int run = wave_incl_scan(sum) - sum;
 #pragma unroll
           for (int q = 0; q < kWpl; ++q) {
-            wpre[lane_id() * kWpl + q] = run;
+            pword[lane_id() * kWpl + q] = make_uint2(pbits[P][lane_id() * kWpl + q], (uint32_t)run);
             run += cw[q];
           }
         }
@@ -1003,19 +1024,22 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
             int x = xb + u * kWave;
             ok[u] = x < n;
             x = ok[u] ? x : n - 1;
-            const uint32_t wb = pbits[P][x >> 5];
-            const int c = cov + wpre[x >> 5] + __popc(wb & ((2u << (x & 31)) - 1u));
+            const uint2 pw = pword[x >> 5];
+            const int c = cov + (int)pw.y + __popc(pw.x & ((2u << (x & 31)) - 1u));
             const SegBias<V> s = sb[c];
             const int64_t q = s.bias + t0 + x;
             item[u] = DenseItem<V>{(uint32_t)bcol[q], bval[q], s.a};
           }
         }
+        // hub columns recur: the bit is tested first, so that mostly unset
+        // bits take the (same-word serialised) atomic OR
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if (ok[u]) {
             const uint32_t k = item[u].key;
-            atomicOr(&bm[k >> 5], 1u << (k & 31));
             atomicAdd(&dense[k], item[u].a * item[u].val);
+            const uint32_t bit = 1u << (k & 31);
+            if (!(bm[k >> 5] & bit)) atomicOr(&bm[k >> 5], bit);
           }
         }
       }
@@ -1030,37 +1054,42 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
       }
     }
     __syncthreads();  // every product of the morsel is in dense/bm
-    // rank the set bits: each thread owns a run of consecutive words
+    // rank the set bits: thread t owns words [t*wpt, t*wpt + wpt); wave
+    // prefix by DPP, one barrier for the wave totals; then every thread
+    // writes its own columns (ascending) straight to C and resets them
     const int wpt = (words + T - 1) / T;
-    const int w0 = threadIdx.x * wpt;
-    const int w1 = (w0 + wpt) < words ? (w0 + wpt) : words;
-    int64_t pc = 0;
-    for (int w = w0; w < w1; ++w) pc += __popc(bm[w]);
-    int64_t tot;
-    int32_t r = (int32_t)block_excl_scan<T, int64_t>(pc, s_w, tot);
+    const int wb0 = threadIdx.x * wpt;
+    const int wb1 = (wb0 + wpt) < words ? (wb0 + wpt) : words;
+    int32_t pc = 0;
+    for (int w = wb0; w < wb1; ++w) pc += __popc(bm[w]);
+    const int32_t incl = wave_incl_scan(pc);
+    if (lane_id() == kWave - 1) s_rank[wave_id()] = incl;
+    __syncthreads();
+    int32_t r = incl - pc, tot = 0;
+#pragma unroll
+    for (int w = 0; w < T / kWave; ++w) {
+      const int32_t v = s_rank[w];
+      r += w < wave_id() ? v : 0;
+      tot += v;
+    }
     if (threadIdx.x == 0 && tot != d.cnt) atomicOr(err, 4);
-    for (int w = w0; w < w1; ++w) {
+    const int32_t colbase = (int32_t)(lo + (int64_t)d.c * W);
+    for (int w = wb0; w < wb1; ++w) {
       uint32_t x = bm[w];
       bm[w] = 0u;
       while (x) {
-        const int b = __ffs(x) - 1;
-        sel[r++] = (uint16_t)((w << 5) + b);
+        const int u = (w << 5) + __ffs(x) - 1;
+        if (r < d.cnt) {
+          col_c[d.off + r] = colbase + u;
+          val_c[d.off + r] = dense[u];
+        }
+        dense[u] = V(0);
+        ++r;
         x &= x - 1;
       }
     }
-    __syncthreads();
-    const int32_t colbase = (int32_t)(lo + (int64_t)d.c * W);
-    const int32_t nout = (int32_t)tot;
-    for (int i = threadIdx.x; i < nout; i += T) {
-      const uint32_t u = sel[i];
-      if (i < d.cnt) {
-        col_c[d.off + i] = colbase + (int32_t)u;
-        val_c[d.off + i] = dense[u];
-      }
-      dense[u] = V(0);
-    }
     d = nd;
-    nd = nnd;
+    ndv = nndv;
     t_cur = t_nxt;
     t_nxt = t_nn;
   }

@@ -253,17 +253,16 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
 }
 
 // Dynamic LDS of k_numc_dense<T>: dense values, column bitmap (padded to
-// 16 bytes), then the segment table of a window or the ranked column list.
+// 16 bytes), then the segment table of a window.
 template <int T>
 static size_t dense_lds(int W, int vbytes) {
   const size_t head = ((size_t)W * vbytes + (size_t)(W / 32) * 4 + 15) & ~(size_t)15;
-  const size_t uni = std::max(sizeof(SegBias<double>) * T, (size_t)W * 2);
-  return head + ((uni + 15) & ~(size_t)15);
+  return head + sizeof(SegBias<double>) * T;
 }
 // Static LDS of k_numc_dense<T> (pass bitmaps, prefix counts, scan and
 // ticket scratch), bounded from above.
 template <int T>
-constexpr size_t dense_static_lds() { return (3 * T * dense_u<T>() / 32 + 2) * 4 + 8 * (T / 64) + 64; }
+constexpr size_t dense_static_lds() { return (4 * T * dense_u<T>() / 32 + 2) * 4 + 12 * (T / 64) + 64; }
 
 // Dense numeric block: 512 threads while two blocks fit a CU's LDS, else
 // 1024 threads so that one block still brings 16 waves.
@@ -725,7 +724,7 @@ void ams_default_params(ams_params_t* p) {
   p->num_hash_max = 2730;
   p->esc_max = 4096;
   p->acc_thres = 2.0;
-  p->dense_width = 7424;
+  p->dense_width = 8640;
   p->bitmap_bits = 1 << 20;
   p->bitmap_ratio = 1.0;
   p->dense_density = 0.05;
@@ -1141,7 +1140,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
     return r;
   }
   const int hash_pairs[][2] = {{682, 682}, {1365, 1365}, {2730, 2730}, {1365, 2730}, {2730, 5461}, {5461, 5461}};
-  const int widths[] = {3968, 7424, 12800, 15360, 24576};
+  const int widths[] = {4288, 8640, 12800, 17344, 24576};
   const double thres[] = {0.0, 1.25, 2.0, 4.0};
   const double ratios[] = {0.25, 1.0, 4.0};
   const double dens[] = {0.02, 0.05, 0.2, 1.0};
