@@ -47,6 +47,8 @@ struct Probs {
   const int64_t* ccol;    // n + 1
   int64_t width0;         // output columns when n <= 1
   int64_t max_width;      // max output columns of one problem
+  const int2* arow_cols;  // per A row: (column base, width) of its problem, or null
+  const int32_t* brow_lo; // per B row: column base of its problem, or null
 };
 
 __device__ __forceinline__ int find_prob(const int64_t* off, int n, int64_t r) {
@@ -62,6 +64,12 @@ __device__ __forceinline__ int find_prob(const int64_t* off, int n, int64_t r) {
 // Column base and width of the problem that owns A row `row`.
 __device__ __forceinline__ void prob_cols(const Probs& pr, int64_t row, int64_t& lo, int64_t& width) {
   if (pr.n <= 1) { lo = 0; width = pr.width0; return; }
+  if (pr.arow_cols) {
+    const int2 v = pr.arow_cols[row];
+    lo = v.x;
+    width = v.y;
+    return;
+  }
   const int pi = find_prob(pr.arow, pr.n, row);
   lo = pr.ccol[pi];
   width = pr.ccol[pi + 1] - lo;
@@ -70,6 +78,7 @@ __device__ __forceinline__ void prob_cols(const Probs& pr, int64_t row, int64_t&
 // Column base of the problem that owns B row k.
 __device__ __forceinline__ int64_t prob_bcol_lo(const Probs& pr, int64_t k) {
   if (pr.n <= 1) return 0;
+  if (pr.brow_lo) return pr.brow_lo[k];
   return pr.ccol[find_prob(pr.brow, pr.n, k)];
 }
 

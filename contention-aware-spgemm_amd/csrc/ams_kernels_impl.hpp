@@ -112,9 +112,56 @@ struct RankGather {
 
 // ------------------------------------------------------------- row flops
 
-// One wave per row of A; also accumulates P into *total.
+// Row flops in groups of 64 rows per wave: rows of at most kFlpShort
+// nonzeros are summed by their own lane, longer ones by the whole wave, one
+// at a time.
+constexpr int64_t kFlpShort = 16;
+// Also accumulates P into *total.
 __global__ __launch_bounds__(256) void k_row_flops(Csr A, Csr B, int64_t* row_flp,
                                                    unsigned long long* total) {
+  __shared__ int64_t s_w[4];
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  int64_t acc = 0;
+  for (int64_t r0 = (blockIdx.x * 4 + wave_id()) * (int64_t)kWave; r0 < A.m; r0 += waves * kWave) {
+    const int64_t r = r0 + lane_id();
+    const bool valid = r < A.m;
+    const int64_t s = valid ? A.rp[r] : 0, e = valid ? A.rp[r + 1] : 0;
+    const bool is_long = e - s > kFlpShort;
+    if (valid && !is_long) {
+      int64_t f = 0;
+      for (int64_t p = s; p < e; ++p) {
+        const int32_t k = A.col[p];
+        f += B.rp[k + 1] - B.rp[k];
+      }
+      if (row_flp) row_flp[r] = f;
+      acc += f;
+    }
+    unsigned long long pending = __ballot(valid && is_long);
+    while (pending) {
+      const int lead = __ffsll(pending) - 1;
+      pending &= pending - 1;
+      const int64_t ss = (int64_t)uniform_u64(__shfl((unsigned long long)s, lead, kWave));
+      const int64_t ee = (int64_t)uniform_u64(__shfl((unsigned long long)e, lead, kWave));
+      int64_t f = 0;
+      for (int64_t p = ss + lane_id(); p < ee; p += kWave) {
+        const int32_t k = A.col[p];
+        f += B.rp[k + 1] - B.rp[k];
+      }
+      f = wave_sum(f);
+      if (lane_id() == 0) {
+        if (row_flp) row_flp[r0 + lead] = f;
+        acc += f;
+      }
+    }
+  }
+  const int64_t t = block_sum<256, int64_t>(acc, s_w);
+  if (threadIdx.x == 0 && t) atomicAdd(total, (unsigned long long)t);
+}
+
+// One wave per row of A (rows longer than kFlpShort on average); also
+// accumulates P into *total.
+__global__ __launch_bounds__(256) void k_row_flops_wave(Csr A, Csr B, int64_t* row_flp,
+                                                        unsigned long long* total) {
   __shared__ int64_t s_w[4];
   const int64_t waves = (int64_t)gridDim.x * 4;
   int64_t acc = 0;
@@ -194,6 +241,56 @@ __device__ __forceinline__ int32_t wave_append(int32_t* counters, int b, bool ac
   return rank;
 }
 
+// Block-aggregated bin counting: lanes add into LDS counters (one LDS
+// atomic per distinct bin per wave), the block adds its totals to the
+// global counters once at the end.  s_cnt holds kMaxBins ints, zeroed by
+// block_count_begin; every thread must call begin and end.
+constexpr int kMaxBins = 32;
+
+__device__ __forceinline__ void block_count_begin(int32_t* s_cnt) {
+  for (int i = threadIdx.x; i < kMaxBins; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ int32_t wave_count_lds(int32_t* s_cnt, int b, bool active) {
+  int32_t rank = -1;
+  unsigned long long pending = __ballot(active);
+  while (pending) {
+    const int lead = __ffsll(pending) - 1;
+    const int bb = __shfl(b, lead, kWave);
+    const unsigned long long mask = __ballot(active && b == bb);
+    int32_t base = 0;
+    if (lane_id() == lead) base = atomicAdd(&s_cnt[bb], (int32_t)__popcll(mask));
+    base = __shfl(base, lead, kWave);
+    if (active && b == bb) {
+      rank = base + (int32_t)__popcll(mask & ((1ull << lane_id()) - 1));
+      active = false;
+    }
+    pending &= ~mask;
+  }
+  return rank;
+}
+
+__device__ __forceinline__ void block_count_end(int32_t* s_cnt, int32_t* counters, int nbins) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+    if (s_cnt[i]) atomicAdd(&counters[i], s_cnt[i]);
+}
+
+// Per-row problem tables of a batch (one binary search per row, once per
+// plan): A row -> (column base, width), B row -> column base.
+__global__ __launch_bounds__(256) void k_prob_rows(Probs pr, int64_t m, int64_t mB, int2* arow_cols,
+                                                   int32_t* brow_lo) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m || i < mB;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < m) {
+      const int pi = find_prob(pr.arow, pr.n, i);
+      arow_cols[i] = make_int2((int32_t)pr.ccol[pi], (int32_t)(pr.ccol[pi + 1] - pr.ccol[pi]));
+    }
+    if (i < mB) brow_lo[i] = (int32_t)pr.ccol[find_prob(pr.brow, pr.n, i)];
+  }
+}
+
 __device__ __forceinline__ int64_t row_width(const Probs& pr, int64_t row) {
   int64_t lo, width;
   prob_cols(pr, row, lo, width);
@@ -204,6 +301,8 @@ __device__ __forceinline__ int64_t row_width(const Probs& pr, int64_t row) {
 __global__ __launch_bounds__(256) void k_classify_sym(int64_t m, const int64_t* row_flp, Probs pr,
                                                       SymParams sp, uint8_t* sbin, int64_t* rownz,
                                                       int32_t* counts) {
+  __shared__ int32_t s_cnt[kMaxBins];
+  block_count_begin(s_cnt);
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = i0 + threadIdx.x;
     const bool valid = i < m;
@@ -223,8 +322,9 @@ __global__ __launch_bounds__(256) void k_classify_sym(int64_t m, const int64_t* 
       }
       sbin[i] = (uint8_t)b;
     }
-    (void)wave_append(counts, b, b != SB_NONE);
+    (void)wave_count_lds(s_cnt, b, b != SB_NONE);
   }
+  block_count_end(s_cnt, counts, SB_BITMAP + 1);
 }
 
 // Numeric route of every row (needs C's row pointers).
@@ -232,6 +332,8 @@ __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* 
                                                       const int64_t* rp_c, const uint8_t* sbin,
                                                       Probs pr, NumParams np, uint8_t* nbin,
                                                       int32_t* counts) {
+  __shared__ int32_t s_cnt[kMaxBins];
+  block_count_begin(s_cnt);
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = i0 + threadIdx.x;
     const bool valid = i < m;
@@ -257,26 +359,34 @@ __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* 
       }
       nbin[i] = (uint8_t)b;
     }
-    (void)wave_append(counts, b, b != NB_NONE);
+    (void)wave_count_lds(s_cnt, b, b != NB_NONE);
   }
+  block_count_end(s_cnt, counts, kNumBins);
 }
 
 // Rows grouped by bin: bin b occupies list[sum(counts[0..b)) ...).
 __global__ __launch_bounds__(256) void k_scatter_bins(int64_t m, const uint8_t* bin, int nbins,
                                                       const int32_t* counts, int32_t* cursors,
                                                       int32_t* list) {
-  __shared__ int32_t s_off[32];
+  __shared__ int32_t s_off[kMaxBins];
+  __shared__ int32_t s_cnt[kMaxBins];
+  __shared__ int32_t s_base[kMaxBins];
   if (threadIdx.x == 0) {
     int32_t acc = 0;
     for (int b = 0; b < nbins; ++b) { s_off[b] = acc; acc += counts[b]; }
   }
-  __syncthreads();
+  // per tile of 256 rows: ranks inside the block, then one global cursor
+  // claim per bin and block
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = i0 + threadIdx.x;
     const int b = i < m ? (int)bin[i] : 0xff;
     const bool active = b != 0xff;
-    const int32_t r = wave_append(cursors, b, active);
-    if (active) list[s_off[b] + r] = (int32_t)i;
+    block_count_begin(s_cnt);
+    const int32_t r = wave_count_lds(s_cnt, b, active);
+    __syncthreads();
+    if ((int)threadIdx.x < nbins) s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&cursors[threadIdx.x], s_cnt[threadIdx.x]) : 0;
+    __syncthreads();
+    if (active) list[s_off[b] + s_base[b] + r] = (int32_t)i;
   }
 }
 
@@ -367,14 +477,69 @@ __global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* li
   }
 }
 
+// Bitmap rows of narrow problems with few products do not fill a block: a
+// wave takes each of them, with a private LDS bitmap of the row's width and
+// no block barriers.  k_symb_bitmap skips these rows.
+constexpr int kWaveBitmapBits = 16384;
+constexpr int64_t kWaveBitmapFlops = 8192;
+
+template <int T>
+__global__ __launch_bounds__(T) void k_symb_bitmap_wave(Csr A, Csr B, Probs pr, const int64_t* row_flp,
+                                                        const int32_t* list, const int32_t* counts, int bin,
+                                                        int64_t* rownz, int32_t* coff, int R, int W) {
+  constexpr int NW = T / kWave;
+  __shared__ uint32_t s_bm[NW][kWaveBitmapBits / 32];
+  __shared__ int32_t s_off[NW][kWave];
+  __shared__ int64_t s_start[NW][kWave];
+  uint32_t* bm = s_bm[wave_id()];
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const SegB seg{A.col, B.rp};
+  const int wchunk = W >> 5;
+  for (int32_t h = blockIdx.x * NW + wave_id(); h < cnt; h += gridDim.x * NW) {
+    const int32_t row = list[start + h];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    if (width > kWaveBitmapBits || row_flp[row] > kWaveBitmapFlops) continue;
+    const int words = (int)((width + 31) >> 5);
+    for (int w = lane_id(); w < words; w += kWave) bm[w] = 0u;
+    wave_lds_sync();
+    wave_expand<8>(A.rp[row], A.rp[row + 1], s_off[wave_id()], s_start[wave_id()], seg,
+                   [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+                   [&](int32_t col) {
+                     const uint32_t u = (uint32_t)((int64_t)col - lo);
+                     const uint32_t bit = 1u << (u & 31);
+                     if (!(bm[u >> 5] & bit)) atomicOr(&bm[u >> 5], bit);
+                   });
+    wave_lds_sync();
+    int32_t* crow = coff + (int64_t)h * (R + 1);
+    const int nch = (int)((width + W - 1) / W);
+    int32_t running = 0;
+    for (int c = 0; c < nch; ++c) {
+      const int w0 = c * wchunk;
+      const int w1 = (w0 + wchunk) < words ? (w0 + wchunk) : words;
+      int32_t pc = 0;
+      for (int w = w0 + lane_id(); w < w1; w += kWave) pc += __popc(bm[w]);
+      pc = wave_sum(pc);
+      if (lane_id() == 0) crow[c] = running;
+      running += pc;
+    }
+    if (lane_id() == 0) {
+      for (int c = nch; c <= R; ++c) crow[c] = running;
+      rownz[row] = running;
+    }
+    wave_lds_sync();  // every lane has read bm before the next row clears it
+  }
+}
+
 // LDS-bitmap symbolic for wide rows, one row per block.  The row's output
 // columns [0, width) are covered in passes of Wb bits; the population of
 // every W-column chunk goes to coff (exclusive prefix, total at index R).
 // Dynamic LDS: Wb/32 words of bitmap + Wb/W chunk counters.
 template <int T>
-__global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int32_t* list,
-                                                   const int32_t* counts, int bin, int64_t* rownz,
-                                                   int32_t* coff, int R, int W, int Wb) {
+__global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
+                                                   const int32_t* list, const int32_t* counts, int bin,
+                                                   int64_t* rownz, int32_t* coff, int R, int W, int Wb) {
   extern __shared__ uint32_t dyn[];
   uint32_t* bm = dyn;
   int32_t* ccount = (int32_t*)(dyn + (Wb >> 5));
@@ -387,6 +552,7 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
     const int32_t row = list[start + h];
     int64_t lo, width;
     prob_cols(pr, row, lo, width);
+    if (width <= kWaveBitmapBits && row_flp[row] <= kWaveBitmapFlops) continue;  // k_symb_bitmap_wave's rows
     int32_t running = 0;
     int32_t* crow = coff + (int64_t)h * (R + 1);
     for (int64_t c0 = 0; c0 < width; c0 += Wb) {
@@ -696,76 +862,260 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
 
 // --------------------------------------------------- column-blocked B
 
-// Entries per (chunk, B row): cnt[c*(mB+1) + k].  One wave per B row.
-// Chunk of an entry = (column - problem column base) / W.
-__global__ __launch_bounds__(64) void k_bblk_count(Csr B, Probs pr, int W, int R, int64_t* cnt) {
-  extern __shared__ int32_t cw[];
-  const int64_t ld = B.m + 1;
-  for (int64_t k = blockIdx.x; k < B.m; k += gridDim.x) {
-    const int64_t lo = prob_bcol_lo(pr, k);
-    for (int c = threadIdx.x; c < R; c += 64) cw[c] = 0;
-    __syncthreads();
-    for (int64_t q = B.rp[k] + threadIdx.x; q < B.rp[k + 1]; q += 64)
-      atomicAdd(&cw[(int)(((int64_t)B.col[q] - lo) / W)], 1);
-    __syncthreads();
-    for (int c = threadIdx.x; c < R; c += 64) cnt[c * ld + k] = cw[c];
-    __syncthreads();
+// Column-blocked copy of B for the dense bin: chunk c of row k holds the
+// entries with (column - problem column base) / W == c.  One wave per B
+// row.  Rows with ascending columns (the usual case) have every chunk as
+// one contiguous run: an entry starts a run when the previous entry lies in
+// another chunk, a DPP max-scan hands every entry its run's start, and the
+// runs are counted / copied without LDS or barriers.  Rows in any other
+// order take a ballot loop over the chunks present in each 64-entry group,
+// with per-chunk cursors in LDS (R ints).
+
+// Lane i - 1's value (wave_shr:1); lane 0 gets `first`.
+__device__ __forceinline__ int32_t wave_prev(int32_t x, int32_t first) {
+  const int32_t y = (int32_t)__builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+  return lane_id() == 0 ? first : y;
+}
+
+// Lane i + 1's value (wave_shl:1); lane 63 gets `last`.
+__device__ __forceinline__ int32_t wave_next(int32_t x, int32_t last) {
+  const int32_t y = (int32_t)__builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+  return lane_id() == kWave - 1 ? last : y;
+}
+
+// True when the columns of B row [s, e) never decrease.
+__device__ __forceinline__ bool row_ascending(const int32_t* col, int64_t s, int64_t e) {
+  bool bad = false;
+  int32_t before = INT32_MIN;  // last column of the previous group
+  for (int64_t q0 = s; q0 < e; q0 += kWave) {
+    const int64_t q = q0 + lane_id();
+    const int32_t x = q < e ? col[q] : INT32_MAX;
+    bad |= wave_prev(x, before) > x;
+    before = __builtin_amdgcn_readlane(x, kWave - 1);
   }
+  return !__any(bad);
+}
+
+// Inclusive prefix max over the 64 lanes of non-negative values.
+__device__ __forceinline__ int32_t wave_incl_max_nonneg(int32_t x) {
+  x = max(x, (int32_t)dpp_move<0x111, 0xf>((uint32_t)x));
+  x = max(x, (int32_t)dpp_move<0x112, 0xf>((uint32_t)x));
+  x = max(x, (int32_t)dpp_move<0x114, 0xf>((uint32_t)x));
+  x = max(x, (int32_t)dpp_move<0x118, 0xf>((uint32_t)x));
+  x = max(x, (int32_t)dpp_move<0x142, 0xa>((uint32_t)x));
+  x = max(x, (int32_t)dpp_move<0x143, 0xc>((uint32_t)x));
+  return x;
+}
+
+// Walks an ascending row in 64-entry groups; for every entry calls
+// f(q, chunk, local column, first index of its run in the row, whether it
+// ends the run).  One 32-bit division per entry; neighbours' chunks come
+// from lane shifts.
+template <class F>
+__device__ __forceinline__ void row_runs(const int32_t* col, int64_t s, int64_t e, int32_t lo, int W, F f) {
+  int32_t carry = 0;   // 1 + row-local index of the latest run start
+  int32_t cprev = -1;  // chunk of the previous group's last entry
+  for (int64_t q0 = s; q0 < e; q0 += kWave) {
+    const int64_t q = q0 + lane_id();
+    const bool valid = q < e;
+    const int32_t loc = valid ? col[q] - lo : 0;
+    const int32_t c = valid ? loc / W : -1;
+    int32_t cn = -1;
+    if (lane_id() == kWave - 1 && q + 1 < e) cn = (col[q + 1] - lo) / W;
+    const int32_t cp = wave_prev(c, cprev);
+    cn = wave_next(c, cn);
+    const bool start = valid && cp != c;
+    int32_t m = wave_incl_max_nonneg(start ? (int32_t)(q - s + 1) : 0);
+    m = m > carry ? m : carry;
+    carry = __builtin_amdgcn_readlane(m, kWave - 1);
+    cprev = __builtin_amdgcn_readlane(c, kWave - 1);
+    if (valid) f(q, c, loc, (int64_t)(m - 1), q + 1 == e || cn != c);
+  }
+}
+
+// Rows of at most kBblkShort entries: one thread per row, walking its
+// entries in order (64 rows in flight per wave).  The wave kernels below
+// skip these rows.
+constexpr int64_t kBblkShort = 64;
+
+__global__ __launch_bounds__(256) void k_bblk_count_short(Csr B, Probs pr, int W, int R, int64_t* cnt,
+                                                          int32_t* lrows, int32_t* nlong) {
+  const int64_t ld = B.m + 1;
+  for (int64_t k0 = blockIdx.x * (int64_t)blockDim.x; k0 < B.m; k0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = k0 + threadIdx.x;
+    const int64_t s = k < B.m ? B.rp[k] : 0, e = k < B.m ? B.rp[k + 1] : 0;
+    // long rows go to the list of the wave-per-row kernels
+    const bool is_long = e - s > kBblkShort;
+    const unsigned long long lm = __ballot(is_long);
+    if (lm) {
+      int32_t base = 0;
+      if (lane_id() == __ffsll(lm) - 1) base = atomicAdd(nlong, (int32_t)__popcll(lm));
+      base = __shfl(base, __ffsll(lm) - 1, kWave);
+      if (is_long) lrows[base + __popcll(lm & ((1ull << lane_id()) - 1))] = (int32_t)k;
+    }
+    if (is_long || e == s) continue;
+    const int32_t lo = (int32_t)prob_bcol_lo(pr, k);
+    int32_t prev_col = INT32_MIN, prev_c = -1, run = 0;
+    bool sorted = true;
+    for (int64_t q = s; q < e; ++q) {
+      const int32_t col = B.col[q];
+      const int32_t c = (col - lo) / W;
+      sorted &= col >= prev_col;
+      prev_col = col;
+      if (c != prev_c) {
+        if (prev_c >= 0) cnt[prev_c * ld + k] = run;
+        prev_c = c;
+        run = 0;
+      }
+      ++run;
+    }
+    if (sorted) {
+      cnt[prev_c * ld + k] = run;
+      continue;
+    }
+    // any other order: recount (this thread alone owns row k's counters)
+    for (int c = 0; c < R; ++c) cnt[c * ld + k] = 0;
+    for (int64_t q = s; q < e; ++q) cnt[((B.col[q] - lo) / W) * ld + k] += 1;
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_bblk_scatter_short(Csr B, Probs pr, int W, const int64_t* bptr,
+                                                            uint16_t* bcol, V* bval, int64_t* perm) {
+  // rows longer than kBblkShort are k_bblk_scatter's
+  const int64_t ld = B.m + 1;
+  const V* bv = (const V*)B.val;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < B.m; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = B.rp[k], e = B.rp[k + 1];
+    if (e - s > kBblkShort || e == s) continue;
+    const int32_t lo = (int32_t)prob_bcol_lo(pr, k);
+    bool sorted = true;
+    for (int64_t q = s + 1; q < e; ++q) sorted &= B.col[q] >= B.col[q - 1];
+    int32_t prev_c = -1;
+    int64_t pos = 0;
+    for (int64_t q = s; q < e; ++q) {
+      const int32_t loc = B.col[q] - lo;
+      const int32_t c = loc / W;
+      if (sorted) {
+        if (c != prev_c) {
+          pos = bptr[c * ld + k];
+          prev_c = c;
+        }
+      } else {  // rank among the row's earlier entries of the same chunk
+        pos = bptr[c * ld + k];
+        for (int64_t j = s; j < q; ++j) pos += ((B.col[j] - lo) / W) == c;
+      }
+      bcol[pos] = (uint16_t)(loc - c * W);
+      bval[pos] = bv[q];
+      if (perm) perm[pos] = q;
+      ++pos;
+    }
+  }
+}
+
+// Long rows (more than kBblkShort entries, listed by k_bblk_count_short):
+// a wave each, persistent over the list.  Per-wave LDS: R chunk cursors for
+// rows whose columns are not ascending.
+template <class Row>
+__device__ __forceinline__ void long_rows(const Csr& B, const int32_t* lrows, const int32_t* nlong, Row row) {
+  const int32_t n = *nlong;
+  const int32_t waves = (int32_t)(gridDim.x * (blockDim.x / kWave));
+  for (int32_t i = (int32_t)(blockIdx.x * (blockDim.x / kWave) + wave_id()); i < n; i += waves) {
+    const int64_t k = lrows[i];
+    row(k, (int64_t)B.rp[k], (int64_t)B.rp[k + 1]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bblk_count(Csr B, Probs pr, int W, int R, int64_t* cnt,
+                                                    const int32_t* lrows, const int32_t* nlong) {
+  extern __shared__ int32_t cw_all[];
+  int32_t* cw = cw_all + (size_t)R * wave_id();
+  const int64_t ld = B.m + 1;
+  long_rows(B, lrows, nlong, [&](int64_t k, int64_t s, int64_t e) {
+    const int32_t lo = (int32_t)prob_bcol_lo(pr, k);
+    if (row_ascending(B.col, s, e)) {
+      // cnt starts at zero; the last entry of every run writes its length
+      row_runs(B.col, s, e, lo, W, [&](int64_t q, int c, int32_t, int64_t first, bool last) {
+        if (last) cnt[c * ld + k] = (q - s) - first + 1;
+      });
+      return;
+    }
+    for (int c = lane_id(); c < R; c += kWave) cw[c] = 0;
+    wave_lds_sync();
+    for (int64_t q = s + lane_id(); q < e; q += kWave) atomicAdd(&cw[(B.col[q] - lo) / W], 1);
+    wave_lds_sync();
+    for (int c = lane_id(); c < R; c += kWave) cnt[c * ld + k] = cw[c];
+    wave_lds_sync();
+  });
 }
 
 // Scatter B into the blocked layout; entries of one (chunk, row) segment
 // keep B's order.  perm (optional) records each entry's original position.
 template <typename V>
-__global__ __launch_bounds__(64) void k_bblk_scatter(Csr B, Probs pr, int W, int R, const int64_t* bptr,
-                                                     uint16_t* bcol, V* bval, int64_t* perm) {
-  extern __shared__ int32_t cw[];
+__global__ __launch_bounds__(256) void k_bblk_scatter(Csr B, Probs pr, int W, int R, const int64_t* bptr,
+                                                      uint16_t* bcol, V* bval, int64_t* perm,
+                                                      const int32_t* lrows, const int32_t* nlong) {
+  extern __shared__ int32_t cw_all[];
+  int32_t* cw = cw_all + (size_t)R * wave_id();
   const int64_t ld = B.m + 1;
   const V* bv = (const V*)B.val;
-  for (int64_t k = blockIdx.x; k < B.m; k += gridDim.x) {
-    const int64_t lo = prob_bcol_lo(pr, k);
-    for (int c = threadIdx.x; c < R; c += 64) cw[c] = 0;
-    __syncthreads();
-    for (int64_t q0 = B.rp[k]; q0 < B.rp[k + 1]; q0 += 64) {
-      const int64_t q = q0 + threadIdx.x;
-      bool active = q < B.rp[k + 1];
+  long_rows(B, lrows, nlong, [&](int64_t k, int64_t s, int64_t e) {
+    const int32_t lo = (int32_t)prob_bcol_lo(pr, k);
+    if (row_ascending(B.col, s, e)) {
+      row_runs(B.col, s, e, lo, W, [&](int64_t q, int c, int32_t loc, int64_t first, bool) {
+        const int64_t pos = bptr[c * ld + k] + (q - s) - first;
+        bcol[pos] = (uint16_t)(loc - c * W);
+        bval[pos] = bv[q];
+        if (perm) perm[pos] = q;
+      });
+      return;
+    }
+    for (int c = lane_id(); c < R; c += kWave) cw[c] = 0;
+    wave_lds_sync();
+    for (int64_t q0 = s; q0 < e; q0 += kWave) {
+      const int64_t q = q0 + lane_id();
+      bool active = q < e;
       int32_t loc = 0;
       int c = 0;
       if (active) {
-        loc = (int32_t)((int64_t)B.col[q] - lo);
+        loc = B.col[q] - lo;
         c = loc / W;
       }
-      unsigned long long pending = __ballot(active);
-      while (pending) {
-        const int lead = __ffsll(pending) - 1;
+      unsigned long long todo = __ballot(active);
+      while (todo) {
+        const int lead = __ffsll(todo) - 1;
         const int cc = __shfl(c, lead, kWave);
         const unsigned long long mask = __ballot(active && c == cc);
         const int32_t prev = cw[cc];
         if (active && c == cc) {
-          const int64_t pos = bptr[cc * ld + k] + prev + __popcll(mask & ((1ull << threadIdx.x) - 1));
+          const int64_t pos = bptr[cc * ld + k] + prev + __popcll(mask & ((1ull << lane_id()) - 1));
           bcol[pos] = (uint16_t)(loc - cc * W);
           bval[pos] = bv[q];
           if (perm) perm[pos] = q;
           active = false;
         }
-        __syncthreads();
-        if (threadIdx.x == lead) cw[cc] = prev + (int32_t)__popcll(mask);
-        __syncthreads();
-        pending &= ~mask;
+        wave_lds_sync();
+        if (lane_id() == lead) cw[cc] = prev + (int32_t)__popcll(mask);
+        wave_lds_sync();
+        todo &= ~mask;
       }
     }
-    __syncthreads();
-  }
+    wave_lds_sync();
+  });
 }
 
 // One morsel of the dense bin: wide row `row`, column chunk `c`.
 struct DenseTask {
   int32_t row;
-  int32_t cnt;   // nnz of C in this chunk
+  int32_t cnt;      // nnz of C in this chunk
   int32_t c;
-  int32_t deg;   // nnz of A's row
-  int64_t off;   // first C slot of the chunk's entries
-  int64_t p0;    // first A nonzero of the row
+  int32_t deg;      // nnz of A's row
+  int64_t off;      // first C slot of the chunk's entries
+  int64_t p0;       // first A nonzero of the row
+  int32_t colbase;  // first output column of the chunk
+  int32_t cwidth;   // output columns in the chunk (<= W)
 };
+static_assert(sizeof(DenseTask) == 40, "descriptor dwords");
 
 // flag[c*H + h] = 1 when morsel (h, c) is routed dense and not empty.
 __global__ __launch_bounds__(256) void k_dense_flags(const int32_t* list, const int32_t* counts, int bin,
@@ -786,9 +1136,10 @@ __global__ __launch_bounds__(256) void k_dense_flags(const int32_t* list, const 
 // Writes the descriptors of the flagged morsels at their scanned positions
 // (pos = exclusive scan of the flags, n_all + 1 entries), keeping the
 // chunk-major order, and the morsel count into *ntask.
-__global__ __launch_bounds__(256) void k_dense_fill(Csr A, const int32_t* list, const int32_t* counts, int bin,
-                                                    const int32_t* coff, int R, const int64_t* rp_c,
-                                                    const int64_t* pos, DenseTask* tasks, int64_t* ntask) {
+__global__ __launch_bounds__(256) void k_dense_fill(Csr A, Probs pr, int W, const int32_t* list,
+                                                    const int32_t* counts, int bin, const int32_t* coff, int R,
+                                                    const int64_t* rp_c, const int64_t* pos, DenseTask* tasks,
+                                                    int64_t* ntask) {
   int32_t start, H;
   bin_range(counts, bin, start, H);
   const int64_t n = (int64_t)H * R;
@@ -806,6 +1157,11 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, const int32_t* list, 
     d.p0 = A.rp[row];
     d.deg = (int32_t)(A.rp[row + 1] - d.p0);
     d.off = rp_c[row] + crow[c];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    d.colbase = (int32_t)(lo + (int64_t)c * W);
+    const int64_t cw = width - (int64_t)c * W;
+    d.cwidth = (int32_t)(cw < W ? cw : W);
     tasks[p] = d;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *ntask = pos[n];
@@ -832,6 +1188,8 @@ __device__ __forceinline__ DenseTask task_get(uint32_t v) {
   d.deg = (int32_t)lane(3);
   d.off = (int64_t)(((uint64_t)lane(5) << 32) | lane(4));
   d.p0 = (int64_t)(((uint64_t)lane(7) << 32) | lane(6));
+  d.colbase = (int32_t)lane(8);
+  d.cwidth = (int32_t)lane(9);
   return d;
 }
 
@@ -946,14 +1304,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   for (unsigned mc = 0; t_cur < ntask; ++mc) {
     // slot (mc+3)&3 was last read at morsel mc-1 and is read next at mc+1;
     // the barriers of every morsel order both
+    // everything waited on here was issued a morsel ago; the new prefetches
+    // go out after it, so that no wait also covers them
+    const DenseTask nd = task_get(ndv);
     const unsigned long long t_nn = uniform_u64(s_ring[(mc + 2) & 3]);
     if (threadIdx.x == 0) s_ring[(mc + 3) & 3] = take();
     const uint32_t nndv = task_load(tasks, t_nn, ntask);  // descriptor two morsels ahead
-    const DenseTask nd = task_get(ndv);
-    int64_t lo, width;
-    prob_cols(pr, d.row, lo, width);
     const int64_t p1 = d.p0 + d.deg;
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
+      // scan of (non-empty count << 32 | length): offsets and compact index
+      int64_t tot64;
+      const int64_t ex = block_excl_scan<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
+      const int32_t off = (int32_t)(ex & 0xffffffffll);
+      const int32_t ci = (int32_t)(ex >> 32);
+      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
       // prefetch the A entries of the next window: this row's next one, or
       // the next morsel's first; their B bounds follow after the passes
       const bool last = w0 + T >= p1;
@@ -975,12 +1339,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
         nk = A.col[np + threadIdx.x];
         na = av[np + threadIdx.x];
       }
-      // scan of (non-empty count << 32 | length): offsets and compact index
-      int64_t tot64;
-      const int64_t ex = block_excl_scan<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
-      const int32_t off = (int32_t)(ex & 0xffffffffll);
-      const int32_t ci = (int32_t)(ex >> 32);
-      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
       if (len > 0) sb[ci] = SegBias<V>{st - off, a};
       for (int32_t t0 = 0; t0 < total; t0 += T * U, ++ps) {
         const int P = ps & 1;
@@ -1057,9 +1415,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     // rank the set bits: thread t owns words [t*wpt, t*wpt + wpt); wave
     // prefix by DPP, one barrier for the wave totals; then every thread
     // writes its own columns (ascending) straight to C and resets them
-    const int wpt = (words + T - 1) / T;
+    // only the words of this chunk's columns can hold bits
+    const int mwords = (d.cwidth + 31) >> 5;
+    const int wpt = (mwords + T - 1) / T;
     const int wb0 = threadIdx.x * wpt;
-    const int wb1 = (wb0 + wpt) < words ? (wb0 + wpt) : words;
+    const int wb1 = (wb0 + wpt) < mwords ? (wb0 + wpt) : mwords;
     int32_t pc = 0;
     for (int w = wb0; w < wb1; ++w) pc += __popc(bm[w]);
     const int32_t incl = wave_incl_scan(pc);
@@ -1073,19 +1433,34 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
       tot += v;
     }
     if (threadIdx.x == 0 && tot != d.cnt) atomicOr(err, 4);
-    const int32_t colbase = (int32_t)(lo + (int64_t)d.c * W);
+    // set bits four at a time: their LDS reads go out together
     for (int w = wb0; w < wb1; ++w) {
       uint32_t x = bm[w];
       bm[w] = 0u;
       while (x) {
-        const int u = (w << 5) + __ffs(x) - 1;
-        if (r < d.cnt) {
-          col_c[d.off + r] = colbase + u;
-          val_c[d.off + r] = dense[u];
+        int u[4];
+        V v[4];
+        int nb = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          u[k] = (w << 5) + __ffs(x) - 1;
+          if (x) ++nb;
+          x &= x - 1;
         }
-        dense[u] = V(0);
-        ++r;
-        x &= x - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k < nb) v[k] = dense[u[k]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (k < nb) {
+            if (r + k < d.cnt) {
+              col_c[d.off + r + k] = d.colbase + u[k];
+              val_c[d.off + r + k] = v[k];
+            }
+            dense[u[k]] = V(0);
+          }
+        }
+        r += nb;
       }
     }
     d = nd;

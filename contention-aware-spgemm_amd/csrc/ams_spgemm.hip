@@ -42,7 +42,8 @@ struct Ctrl {
   unsigned long long flops;
   unsigned long long ticket;
   int64_t ntask;
-  int64_t pad1;
+  int32_t nlong;  // long B rows listed for the wave-per-row blocked copy
+  int32_t pad1;
   unsigned long long sums[16];
   int64_t pad2[5];
 };
@@ -66,7 +67,7 @@ struct ams_ctx {
   ams_stats_t stats{};
   std::string err;
   Buf row_flp, sbin, nbin, lsym, lnum, ctrl, coff, bptr, bcol, bval, bperm, tsum;
-  Buf pool_rp, pool_col, pool_val, probs, scratch_a, dflag, dtask, ccnt;
+  Buf pool_rp, pool_col, pool_val, probs, prow, pbrow, scratch_a, dflag, dtask, ccnt, lrows;
   bool bblk_ready = false;  // blocked B of the current plan already built
   Ctrl* h_ctrl = nullptr;       // pinned mirror of the control block
   int64_t* h_pin = nullptr;     // pinned scratch
@@ -285,8 +286,13 @@ static int launch_row_flops(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t*
     hipLaunchKernelGGL(k_row_flops_nz, dim3(g), dim3(256), 0, ctx->stream, A, B, (unsigned long long*)row_flp,
                        total);
   } else {
-    const int g = (int)std::min<int64_t>((A.m + 3) / 4, (int64_t)ctx->num_cu * 16);
-    hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, row_flp, total);
+    if (A.nnz <= kFlpShort * A.m) {  // short rows: a lane each
+      const int g = (int)std::min<int64_t>((A.m + 255) / 256, (int64_t)ctx->num_cu * 16);
+      hipLaunchKernelGGL(k_row_flops, dim3(g), dim3(256), 0, ctx->stream, A, B, row_flp, total);
+    } else {
+      const int g = (int)std::min<int64_t>((A.m + 3) / 4, (int64_t)ctx->num_cu * 16);
+      hipLaunchKernelGGL(k_row_flops_wave, dim3(g), dim3(256), 0, ctx->stream, A, B, row_flp, total);
+    }
   }
   HIPCHK(hipGetLastError());
   return AMS_OK;
@@ -303,13 +309,26 @@ static int build_bblk(ams_ctx_t* ctx, const Csr& B, const Probs& pr, int W, int 
   int64_t* bptr = (int64_t*)ctx->bptr.p;
   CHK(fill_i64(ctx, bptr, (int64_t)R * ld, 0));
   if (B.m > 0) {
-    const int g = (int)std::min<int64_t>(B.m, (int64_t)ctx->num_cu * 32);
-    const size_t dyn = sizeof(int32_t) * R;
-    hipLaunchKernelGGL(k_bblk_count, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R, bptr);
+    const int g = ctx->num_cu * 8;  // persistent waves over the long-row list
+    const size_t dyn = sizeof(int32_t) * R * 4;  // per wave of a 256-thread block
+    if (dyn > ctx->lds_max)
+      return fail(ctx, AMS_ERR_ARG, "%d column chunks of width %d exceed the LDS cursors; raise dense_width", R, W);
+    const int gs = (int)std::min<int64_t>((B.m + 255) / 256, (int64_t)ctx->num_cu * 8);
+    CHK(grow(ctx, ctx->lrows, sizeof(int32_t) * B.m));
+    Ctrl* c = dctrl(ctx);
+    CHK(fill_i32(ctx, &c->nlong, 1, 0));
+    int32_t* lrows = (int32_t*)ctx->lrows.p;
+    hipLaunchKernelGGL(k_bblk_count_short, dim3(gs), dim3(256), 0, ctx->stream, B, pr, W, R, bptr, lrows,
+                       &c->nlong);
+    hipLaunchKernelGGL(k_bblk_count, dim3(g), dim3(256), dyn, ctx->stream, B, pr, W, R, bptr,
+                       (const int32_t*)lrows, (const int32_t*)&c->nlong);
     CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
-    hipLaunchKernelGGL(k_bblk_scatter<V>, dim3(g), dim3(64), dyn, ctx->stream, B, pr, W, R,
-                       (const int64_t*)bptr, (uint16_t*)ctx->bcol.p, (V*)ctx->bval.p,
-                       perm ? (int64_t*)ctx->bperm.p : (int64_t*)nullptr);
+    int64_t* bperm = perm ? (int64_t*)ctx->bperm.p : (int64_t*)nullptr;
+    hipLaunchKernelGGL(k_bblk_scatter_short<V>, dim3(gs), dim3(256), 0, ctx->stream, B, pr, W,
+                       (const int64_t*)bptr, (uint16_t*)ctx->bcol.p, (V*)ctx->bval.p, bperm);
+    hipLaunchKernelGGL(k_bblk_scatter<V>, dim3(g), dim3(256), dyn, ctx->stream, B, pr, W, R,
+                       (const int64_t*)bptr, (uint16_t*)ctx->bcol.p, (V*)ctx->bval.p, bperm,
+                       (const int32_t*)lrows, (const int32_t*)&c->nlong);
     HIPCHK(hipGetLastError());
   } else {
     CHK(scan_i64(ctx, bptr, bptr, (int64_t)R * ld));
@@ -437,9 +456,19 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
       occupancy(k_symb_bitmap<kSymBitmapT>, kSymBitmapT, dyn, &occ);
       const int g = grid_for(ctx, occ, H, ctx->plan_counts);
       hipLaunchKernelGGL(k_symb_bitmap<kSymBitmapT>, dim3(g), dim3(kSymBitmapT), dyn, ctx->stream, A, B, pr,
-                         (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
-                         (int32_t*)ctx->coff.p, R, W, Wb);
+                         (const int64_t*)ctx->row_flp.p, (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym,
+                         (int)SB_BITMAP, rownz, (int32_t*)ctx->coff.p, R, W, Wb);
       HIPCHK(hipGetLastError());
+      if (pr.max_width <= kWaveBitmapBits) {  // light rows of narrow problems: a wave each
+        constexpr int TW = 256;
+        int occw;
+        occupancy(k_symb_bitmap_wave<TW>, TW, 0, &occw);
+        const int gw = grid_for(ctx, occw, (H + TW / 64 - 1) / (TW / 64), ctx->plan_counts);
+        hipLaunchKernelGGL(k_symb_bitmap_wave<TW>, dim3(gw), dim3(TW), 0, ctx->stream, A, B, pr,
+                           (const int64_t*)ctx->row_flp.p, (const int32_t*)ctx->lsym.p,
+                           (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz, (int32_t*)ctx->coff.p, R, W);
+        HIPCHK(hipGetLastError());
+      }
     }
   }
   {
@@ -576,7 +605,7 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
                          (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const uint8_t*)ctx->nbin.p,
                          (const int32_t*)ctx->coff.p, R, flag);
       CHK(scan_i64(ctx, flag, flag, nall + 1));
-      hipLaunchKernelGGL(k_dense_fill, dim3(gf), dim3(256), 0, ctx->stream, A, (const int32_t*)ctx->lsym.p,
+      hipLaunchKernelGGL(k_dense_fill, dim3(gf), dim3(256), 0, ctx->stream, A, pr, W, (const int32_t*)ctx->lsym.p,
                          (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const int32_t*)ctx->coff.p, R, rp_c,
                          (const int64_t*)flag, (DenseTask*)ctx->dtask.p, &c->ntask);
       HIPCHK(hipGetLastError());
@@ -1031,6 +1060,16 @@ int ams_batched_plan(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, i
     pr.arow = d;
     pr.brow = d + (nprob + 1);
     pr.ccol = d + 2 * (nprob + 1);
+    // per-row problem tables: kernels look a row's problem up with one load
+    CHK(grow(ctx, ctx->prow, sizeof(int2) * std::max<int64_t>(a->num_row, 1)));
+    CHK(grow(ctx, ctx->pbrow, sizeof(int32_t) * std::max<int64_t>(b->num_row, 1)));
+    const int64_t mm = std::max<int64_t>(a->num_row, b->num_row);
+    const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((mm + 255) / 256, (int64_t)ctx->num_cu * 16));
+    hipLaunchKernelGGL(k_prob_rows, dim3(gp), dim3(256), 0, ctx->stream, pr, a->num_row, b->num_row,
+                       (int2*)ctx->prow.p, (int32_t*)ctx->pbrow.p);
+    HIPCHK(hipGetLastError());
+    pr.arow_cols = (const int2*)ctx->prow.p;
+    pr.brow_lo = (const int32_t*)ctx->pbrow.p;
   }
   const int64_t m = a->num_row;
   CHK(grow(ctx, ctx->pool_rp, sizeof(int64_t) * (m + 1)));
