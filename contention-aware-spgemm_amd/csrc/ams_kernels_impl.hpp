@@ -1412,57 +1412,47 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
       }
     }
     __syncthreads();  // every product of the morsel is in dense/bm
-    // rank the set bits: thread t owns words [t*wpt, t*wpt + wpt); wave
-    // prefix by DPP, one barrier for the wave totals; then every thread
-    // writes its own columns (ascending) straight to C and resets them
-    // only the words of this chunk's columns can hold bits
+    // rank and emit by waves: wave w owns words [w*wpw, w*wpw + wpw) of this
+    // chunk's bitmap; its popcount goes through LDS (one barrier), then the
+    // wave walks its words two at a time, one bit per lane: set bits get
+    // their rank by ballot, so each step stores a coalesced run of C
     const int mwords = (d.cwidth + 31) >> 5;
-    const int wpt = (mwords + T - 1) / T;
-    const int wb0 = threadIdx.x * wpt;
-    const int wb1 = (wb0 + wpt) < mwords ? (wb0 + wpt) : mwords;
+    const int wpw = (mwords + T / kWave - 1) / (T / kWave);
+    const int wv0 = wave_id() * wpw;
+    const int wv1 = (wv0 + wpw) < mwords ? (wv0 + wpw) : mwords;
     int32_t pc = 0;
-    for (int w = wb0; w < wb1; ++w) pc += __popc(bm[w]);
-    const int32_t incl = wave_incl_scan(pc);
-    if (lane_id() == kWave - 1) s_rank[wave_id()] = incl;
+    for (int w = wv0 + lane_id(); w < wv1; w += kWave) pc += __popc(bm[w]);
+    pc = wave_sum(pc);
+    if (lane_id() == 0) s_rank[wave_id()] = pc;
     __syncthreads();
-    int32_t r = incl - pc, tot = 0;
+    int32_t base = 0, tot = 0;
 #pragma unroll
     for (int w = 0; w < T / kWave; ++w) {
       const int32_t v = s_rank[w];
-      r += w < wave_id() ? v : 0;
+      base += w < wave_id() ? v : 0;
       tot += v;
     }
     if (threadIdx.x == 0 && tot != d.cnt) atomicOr(err, 4);
-    // set bits four at a time: their LDS reads go out together
-    for (int w = wb0; w < wb1; ++w) {
-      uint32_t x = bm[w];
-      bm[w] = 0u;
-      while (x) {
-        int u[4];
-        V v[4];
-        int nb = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          u[k] = (w << 5) + __ffs(x) - 1;
-          if (x) ++nb;
-          x &= x - 1;
+    const int half = lane_id() >> 5, bit = lane_id() & 31;
+    const unsigned long long below = (1ull << lane_id()) - 1;
+    for (int w = wv0; w < wv1; w += 2) {
+      const int ww = w + half;
+      const uint32_t x = ww < wv1 ? bm[ww] : 0u;
+      const bool set = (x >> bit) & 1u;
+      const unsigned long long mask = __ballot(set);
+      if (set) {
+        const int32_t r = base + (int32_t)__popcll(mask & below);
+        const int u = (ww << 5) + bit;
+        if (r < d.cnt) {
+          col_c[d.off + r] = d.colbase + u;
+          val_c[d.off + r] = dense[u];
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (k < nb) v[k] = dense[u[k]];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (k < nb) {
-            if (r + k < d.cnt) {
-              col_c[d.off + r + k] = d.colbase + u[k];
-              val_c[d.off + r + k] = v[k];
-            }
-            dense[u[k]] = V(0);
-          }
-        }
-        r += nb;
+        dense[u] = V(0);
       }
+      base += (int32_t)__popcll(mask);
     }
+    wave_lds_sync();  // the wave's reads of its words are done
+    for (int w = wv0 + lane_id(); w < wv1; w += kWave) bm[w] = 0u;
     d = nd;
     ndv = nndv;
     t_cur = t_nxt;

@@ -80,6 +80,18 @@ def test_rectangular_unsorted(gpu_ctx, route):
     assert_matches_oracle(c, a, b, exact=False)
 
 
+@pytest.mark.parametrize("width", [8640, 14848])
+def test_wide_chunks(gpu_ctx, width):
+    """Chunks wide enough for the 1024-thread dense kernel (14848 columns)
+    next to the default 8640-column chunks, on a 16384-column problem."""
+    a = gen.rmat_symmetric(14, 2, seed=77)
+    print(f"[parity] rmat_s14_ef2 dense_width={width}", flush=True)
+    c = run(gpu_ctx, a, a, dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_density=0.0,
+                                dense_width=width))
+    assert c.sorted_rows()
+    assert_matches_oracle(c, a, a, exact=True)
+
+
 def test_edge_shapes(gpu_ctx):
     z = Csr(5, 7, np.zeros(6, np.int64), np.zeros(0, np.int32), np.zeros(0))
     b = random_csr(7, 9, 0.5, seed=4)
