@@ -100,6 +100,26 @@ def read_traffic(kernel, workload):
     return None
 
 
+def host_cpu():
+    """Host CPU model and SMT state, from /proc/cpuinfo."""
+    model, siblings, cores = "unknown", None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "siblings" and siblings is None:
+                    siblings = int(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+    except OSError:
+        pass
+    smt = "unknown" if not (siblings and cores) else ("on" if siblings > cores else "off")
+    return f"{model}; {os.cpu_count()} logical CPUs visible; SMT {smt}"
+
+
 def cpu_baseline(a, b, target_s, threads):
     """The oracle on every s-th row of A (same B), s chosen so the timed run
     takes about target_s seconds on `threads` host threads."""
@@ -138,6 +158,7 @@ def cpu_baseline(a, b, target_s, threads):
     wall = time.perf_counter() - t0
     t = rt["total_time"]
     return {"value": 2.0 * ps / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "host": host_cpu(),
             "sample": f"every {stride}th row of A ({sub.num_row} rows, {ps} products = "
                       f"{100.0 * ps / total:.2f}% of P), same B; oracle total_time {t:.2f} s "
                       f"(wall {wall:.2f} s) on {threads} threads; AMS_PRT 0.85, AMS_ACC_THRES 2.0, "
@@ -151,7 +172,7 @@ def bench_batched(count, seed, reps):
     import torch
 
     from ams_hip import gen
-    from ams_hip.batched import BatchedProduct
+    from ams_hip.batched import BatchedProduct, Stacked, autotune_batched
     from ams_hip.device import Context, DeviceCsr
 
     t0 = time.perf_counter()
@@ -160,7 +181,12 @@ def bench_batched(count, seed, reps):
     P = sum(int((np.diff(a.row_ptrs).astype(np.int64) ** 2).sum()) for a in probs)
     stream = torch.cuda.Stream()
     ctx = Context(stream=stream)
-    bp = BatchedProduct([(a, a) for a in probs], ams=ctx)
+    pairs = [(a, a) for a in probs]
+    stacked = Stacked(pairs)
+    t0 = time.perf_counter()
+    tuned, tuned_ms = autotune_batched(pairs, ctx, reps=2, stacked=stacked)
+    log(f"batched autotune {time.perf_counter() - t0:.1f} s: {tuned_ms:.2f} ms with {tuned}")
+    bp = BatchedProduct(pairs, ams=ctx, stacked=stacked)
     bp.run()
     bp.synchronize()
     t0 = time.perf_counter()
@@ -188,7 +214,10 @@ def bench_batched(count, seed, reps):
     return {"problems": count, "flops_P": P, "nnz_c": nnz_batched, "nnz_c_plain_sum": nnz_plain,
             "batched_ms": t_batched * 1e3, "plain_sum_ms": t_plain * 1e3,
             "speedup_vs_plain_launches": t_plain / t_batched,
-            "batched_gflops": 2.0 * P / t_batched / 1e9, "graph_replays": reps}
+            "batched_gflops": 2.0 * P / t_batched / 1e9, "graph_replays": reps,
+            "autotuned": {k: tuned[k] for k in ("sym_hash_max", "num_hash_max", "acc_thres", "dense_width",
+                                                "bitmap_ratio", "dense_density", "esc_sort")},
+            "plain_params": "library defaults"}
 
 
 def bench_fwd_bwd(scale, ef, reps):
@@ -282,9 +311,13 @@ def main():
         log(f"warmup {i} done")
     ctx.enable_timing(True)
     torch.cuda.synchronize()
+    step_ms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        c = ctx.spgemm_pooled(da, db)
+        ts = time.perf_counter()
+        c = ctx.spgemm_pooled(da, db)  # one-shot: ends with the nnz readback
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     tm = ctx.timings()
@@ -334,7 +367,8 @@ def main():
 
     line = {
         "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": ms_step, "ms_per_step_median": float(np.median(step_ms)),
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded R-MAT / incidence generator)",
         "config": {"workload": WORKLOADS[args.workload]["text"], "name": args.workload, "m": a.num_row,
                    "n": b.num_col, "nnz_a": a.num_nnz, "flops_P": P, "nnz_c": st["nnz_c"],

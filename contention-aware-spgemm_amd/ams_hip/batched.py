@@ -47,17 +47,29 @@ def stack(problems: List[Tuple[Csr, Csr]]):
     return A, B, np.array(a_row, np.int64), np.array(b_row, np.int64), np.array(c_col, np.int64)
 
 
-class BatchedProduct:
-    """Plans a batch once; ``run()`` replays one captured graph."""
+class Stacked:
+    """The block-diagonal stack of a batch, uploaded once (reused by every
+    plan of the autotuner)."""
 
-    def __init__(self, problems: List[Tuple[Csr, Csr]], ams: Context | None = None, capture: bool = True):
-        self.problems = problems
-        self.stream = torch.cuda.Stream() if ams is None else ams.stream
-        self.ams = ams or Context(stream=self.stream)
+    def __init__(self, problems: List[Tuple[Csr, Csr]]):
         A, B, self.a_off, self.b_off, self.c_off = stack(problems)
         self.A = DeviceCsr.from_host(A)
         self.B = DeviceCsr.from_host(B)
-        torch.cuda.synchronize()  # uploads ran on the current stream, the plan runs on self.stream
+        torch.cuda.synchronize()  # uploads ran on the current stream; plans run on the context's
+
+
+class BatchedProduct:
+    """Plans a batch once; ``run()`` replays one captured graph."""
+
+    def __init__(self, problems: List[Tuple[Csr, Csr]], ams: Context | None = None, capture: bool = True,
+                 stacked: "Stacked | None" = None):
+        self.problems = problems
+        self.stream = torch.cuda.Stream() if ams is None else ams.stream
+        self.ams = ams or Context(stream=self.stream)
+        stacked = stacked or Stacked(problems)
+        self.a_off, self.b_off, self.c_off = stacked.a_off, stacked.b_off, stacked.c_off
+        self.A, self.B = stacked.A, stacked.B
+        A = stacked.A
         lib = L.lib()
         sa, sb = self.A.c_struct(), self.B.c_struct()
         cap = C.c_int64()
@@ -92,7 +104,10 @@ class BatchedProduct:
                           "ams_batched_run")
 
     def synchronize(self):
-        self.stream.synchronize()
+        if self.stream is None:
+            torch.cuda.synchronize()
+        else:
+            self.stream.synchronize()
 
     def results(self) -> List[Csr]:
         """Per-problem products (host copies, columns rebased)."""
@@ -119,3 +134,59 @@ class BatchedProduct:
             self.close()
         except Exception:
             pass
+
+
+# Coordinate search of the batched autotuner: each parameter in turn over
+# its candidates, keeping the best so far (the same space as ams_autotune).
+TUNE_SPACE = [
+    ("hash", [(682, 682), (1365, 1365), (2730, 2730), (1365, 2730), (2730, 5461), (5461, 5461)]),
+    ("dense_width", [4288, 8640, 12800, 17344]),
+    ("acc_thres", [0.0, 1.25, 2.0, 4.0]),
+    ("bitmap_ratio", [0.25, 1.0, 4.0]),
+    ("dense_density", [0.02, 0.05, 0.2, 1.0]),
+    ("esc_sort", [0, 1]),
+]
+
+
+def autotune_batched(problems: List[Tuple[Csr, Csr]], ams: Context, reps: int = 2, stacked: "Stacked | None" = None):
+    """Picks routing thresholds and LDS geometry for a batch by timing the
+    planned, unsynchronised batched run (plan outside the timing) over the
+    coordinate search in TUNE_SPACE.  Leaves the best parameters set on
+    `ams` and returns (params dict, ms per batched run)."""
+    stacked = stacked or Stacked(problems)
+
+    def timed(params):
+        ams.set_params(**params)
+        bp = BatchedProduct(problems, ams=ams, capture=False, stacked=stacked)
+        bp.run()
+        bp.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        stream = bp.stream if bp.stream is not None else torch.cuda.current_stream()
+        t0.record(stream)
+        for _ in range(reps):
+            bp.run()
+        t1.record(stream)
+        bp.synchronize()
+        bp.close()
+        return t0.elapsed_time(t1) / reps
+
+    best = ams.params()
+    best_ms = timed(best)
+    for name, cands in TUNE_SPACE:
+        for cand in cands:
+            p = dict(best)
+            if name == "hash":
+                p["sym_hash_max"], p["num_hash_max"] = cand
+            else:
+                p[name] = cand
+                if name == "dense_width" and p["bitmap_bits"] < cand:
+                    p["bitmap_bits"] = cand
+            try:
+                ms = timed(p)
+            except L.AmsError:
+                continue  # outside this batch's limits (LDS, table sizes)
+            if ms < best_ms:
+                best, best_ms = p, ms
+    ams.set_params(**best)
+    return best, best_ms

@@ -66,3 +66,26 @@ def test_batched_matches_plain(gpu_ctx):
         np.testing.assert_allclose(c.rats, p.rats, rtol=1e-12, atol=1e-13)
     torch.cuda.synchronize()
     bp.close()
+
+
+def test_batched_autotune_keeps_results(gpu_ctx):
+    """The batched autotuner times planned runs over its search space, leaves
+    the best parameters on the context, and the batch planned with them
+    still matches the oracle."""
+    import torch
+    from ams_hip.batched import BatchedProduct, Stacked, autotune_batched
+    from ams_hip.device import Context
+    probs = small_batch(12, seed=21)
+    stream = torch.cuda.Stream()
+    ctx = Context(stream=stream)
+    stacked = Stacked(probs)
+    best, ms = autotune_batched(probs, ctx, reps=1, stacked=stacked)
+    assert ms > 0
+    assert ctx.params() == best
+    bp = BatchedProduct(probs, ams=ctx, stacked=stacked)
+    bp.run()
+    for (a, b), c in zip(probs, bp.results()):
+        assert c.sorted_rows()
+        assert_matches_oracle(c, a, b, exact=False)
+    bp.close()
+    ctx.close()
