@@ -257,6 +257,30 @@ def bench_fwd_bwd(scale, ef, reps):
             "fwd_gflops": 2.0 * P / t_f / 1e9}
 
 
+def bench_mtx_io(scale, ef):
+    """MatrixMarket round trip of a symmetric R-MAT structure through the
+    library's writer and reader (host only)."""
+    import tempfile
+
+    from ams_hip import gen
+    from ams_hip.mtx import read_mtx, write_mtx
+
+    a = gen.rmat_symmetric(scale, ef, gen.G500, seed=SEED + scale)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "a.mtx")
+        t0 = time.perf_counter()
+        write_mtx(path, a, symmetric=True)
+        t_w = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        back = read_mtx(path)
+        t_r = time.perf_counter() - t0
+    exact = bool(np.array_equal(back.row_ptrs, a.row_ptrs) and np.array_equal(back.cols, a.cols))
+    return {"workload": f"G500 R-MAT s{scale} ef{ef} symmetric MatrixMarket", "file_bytes": size,
+            "entries": a.num_nnz, "write_s": t_w, "read_s": t_r, "read_mb_per_s": size / t_r / 1e6,
+            "round_trip_exact": exact}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +388,8 @@ def main():
         log("fwd+bwd section")
         extras["fwd_bwd"] = bench_fwd_bwd(18, 16, 3)
         log(f"fwd+bwd: {extras['fwd_bwd']}")
+        extras["mtx_io"] = bench_mtx_io(16, 16)
+        log(f"mtx io: {extras['mtx_io']}")
 
     line = {
         "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,

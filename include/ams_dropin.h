@@ -19,6 +19,7 @@
 #define AMS_DROPIN_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -96,6 +97,33 @@ my_cnt_t get_intprod(csr_t *csr_a, csr_t *csr_b, my_cnt_t *row_nz);
 void flp_populate(csr_t *csr_a, csr_t *csr_b, my_cnt_t *row_flp);
 /* src/mm/mm_utils.h:111 (src/mm/mm_utils.c:963-974) */
 my_cnt_t *lower_bound(my_cnt_t *start, my_cnt_t *end, my_cnt_t val);
+
+/* src/inc/store.h:22-26 */
+typedef struct {
+	my_cnt_t user_id;
+	my_cnt_t item_id;
+	my_rat_t rating;
+} triple_t;
+
+/* MatrixMarket / PaRMAT text I/O (ams_mtx.c), src/mm/mm_utils.h:75-101 with
+ * the file semantics of src/mm/mm_utils.c:451-862.  Triple arrays are
+ * mmap'ed; release them with dealloc_memory(ptr, bytes) (src/inc/utils.c:254-260). */
+void dealloc_memory(void *ptr, size_t size);
+int compare_triples(const void *a, const void *b);
+bool mtx2trpl(triple_t **trpl, my_cnt_t *num_row, my_cnt_t *num_col, my_cnt_t *num_nnz, const char *filename,
+	const int numa_node_idx);
+void trpl2mtx(const char *filename, triple_t *trpl, my_cnt_t num_row, my_cnt_t num_col, my_cnt_t num_nnz,
+	bool isSymmetric);
+void trpl2csr(csr_t *csr, triple_t *trpl, const my_cnt_t num_row, const my_cnt_t num_col, const my_cnt_t num_nnz,
+	numa_cfg_t *numa_cfg);
+void mtx2csr(csr_t *csr, const char *filename, numa_cfg_t *numa_cfg);
+void trpl_ones(triple_t *trpl, my_cnt_t num_nnz);
+void trpl_transpose(triple_t *trpl, my_cnt_t *num_row, my_cnt_t *num_col, my_cnt_t num_nnz);
+void txt2trpl(triple_t **trpl, my_cnt_t *num_row, my_cnt_t *num_col, my_cnt_t *num_nnz, const char *filename,
+	const int numa_node_idx);
+/* src/rmat_ones.c:6-37: PaRMAT text -> symmetric all-ones .org.mtx and its transpose .trs.mtx */
+void txt_transpose(const char *filename_dest_trs, const char *filename_dest_org, const char *filename_src,
+	const int numa_node_idx);
 
 /* src/ams/ams_utils.h:60-94 */
 void free_ams_pack(ams_pack_t *ams_pack);
