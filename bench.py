@@ -41,6 +41,9 @@ WORKLOADS = {
                              "C = A*A^T"),
     "g500_s16_ef16": dict(kind="rmat", scale=16, ef=16, abc="G500",
                           text="G500 R-MAT s16 ef16, symmetric, loop/dup-free, FP64 1.0, C = A*A^T"),
+    "parmat_g500_s19_ef32": dict(kind="parmat", scale=19, ef=32,
+                                 text="G500 s19 ef32 with PaRMAT's construction (sorted, undirected, "
+                                      "loop/dup-free, 8 columns of squares), mirrored, FP64 1.0, C = A*A^T"),
     "join_agg": dict(kind="join",
                      text="join-aggregate incidence 850000 x 20000 (lognormal(2,1.5) degrees capped 5000, "
                           "Zipf(1.2) columns), FP64 1.0, C = A^T*A"),
@@ -59,6 +62,10 @@ def make_inputs(name):
         abc = gen.G500 if w["abc"] == "G500" else gen.ER
         a = gen.rmat_symmetric(w["scale"], w["ef"], abc, seed=SEED + w["scale"])
         return a, a  # symmetric: A^T = A
+    if w["kind"] == "parmat":
+        from ams_hip import parmat
+        a = parmat.parmat_symmetric(w["scale"], w["ef"], gen.G500, seed=SEED + w["scale"])
+        return a, a
     inc = gen.join_incidence(seed=SEED)
     return inc.transpose(), inc  # C = A^T A: csr_a = A^T, csr_b = A
 
@@ -257,6 +264,33 @@ def bench_fwd_bwd(scale, ef, reps):
             "fwd_gflops": 2.0 * P / t_f / 1e9}
 
 
+def bench_workload(name, reps):
+    """A few one-shot products of another workload (GFLOP/s = 2P/t)."""
+    import torch
+
+    from ams_hip.device import Context, DeviceCsr
+    t0 = time.perf_counter()
+    a, b = make_inputs(name)
+    gen_s = time.perf_counter() - t0
+    da = DeviceCsr.from_host(a)
+    db = da if b is a else DeviceCsr.from_host(b)
+    ctx = Context()
+    _, P = ctx.row_flops(da, db)
+    c = ctx.spgemm_pooled(da, db)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c = ctx.spgemm_pooled(da, db)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    ver = ctx.verify(c, 0)
+    out = {"workload": WORKLOADS[name]["text"], "nnz_a": a.num_nnz, "flops_P": P, "nnz_c": ver["nnz"],
+           "ms_per_step": ms, "gflops": 2.0 * P / (ms * 1e-3) / 1e9, "sum_val_equals_P": ver["sum_val"] == float(P),
+           "unsorted_rows": ver["unsorted_rows"], "generate_s": gen_s}
+    del ctx
+    return out
+
+
 def bench_mtx_io(scale, ef):
     """MatrixMarket round trip of a symmetric R-MAT structure through the
     library's writer and reader (host only)."""
@@ -390,12 +424,15 @@ def main():
         log(f"fwd+bwd: {extras['fwd_bwd']}")
         extras["mtx_io"] = bench_mtx_io(16, 16)
         log(f"mtx io: {extras['mtx_io']}")
+        if args.workload != "parmat_g500_s19_ef32":
+            extras["parmat_g500_s19_ef32"] = bench_workload("parmat_g500_s19_ef32", 3)
+            log(f"parmat: {extras['parmat_g500_s19_ef32']}")
 
     line = {
         "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "ms_per_step_median": float(np.median(step_ms)),
         "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded R-MAT / incidence generator)",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded R-MAT / PaRMAT-construction / incidence generators)",
         "config": {"workload": WORKLOADS[args.workload]["text"], "name": args.workload, "m": a.num_row,
                    "n": b.num_col, "nnz_a": a.num_nnz, "flops_P": P, "nnz_c": st["nnz_c"],
                    "params": params_used},
