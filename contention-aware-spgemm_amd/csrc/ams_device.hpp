@@ -204,6 +204,17 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
   }
 }
 
+// Reads an LDS accumulator and leaves 0 in it, in one exchange.
+template <typename V>
+__device__ __forceinline__ V lds_take(V* p) {
+  static_assert(sizeof(V) == 4 || sizeof(V) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(V) == 8) {
+    return __builtin_bit_cast(V, atomicExch((unsigned long long*)p, 0ull));
+  } else {
+    return __builtin_bit_cast(V, atomicExch((unsigned int*)p, 0u));
+  }
+}
+
 // Makes this wave's LDS writes visible to its other lanes.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -1227,8 +1227,8 @@ struct DenseItem {
 // barriers.  The next morsel's descriptor, A columns and B segment bounds
 // are loaded while the current one runs.
 //
-// Dynamic LDS: V dense[W] | uint32 bitmap[W/32] (padded to 16 bytes) |
-//              SegBias<V> seg[T]
+// Dynamic LDS: V dense[W + 64] | uint32 bitmap[(W + 64)/32] (padded to 16
+//              bytes) | SegBias<V> seg[T]; entry W + lane is lane's sink
 #ifndef AMS_DENSE_WPE
 #define AMS_DENSE_WPE 4
 #endif
@@ -1246,9 +1246,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   constexpr int kWpl = kPassWords / kWave;  // pass bitmap words per lane of wave 0
   static_assert(kPassWords % kWave == 0, "one wave scans the pass bitmap");
   extern __shared__ __align__(16) unsigned char dyn_raw[];
-  V* dense = (V*)dyn_raw;
-  uint32_t* bm = (uint32_t*)(dense + W);
-  unsigned char* uni = dyn_raw + (((size_t)W * sizeof(V) + (size_t)(W >> 5) * 4 + 15) & ~(size_t)15);
+  V* dense = (V*)dyn_raw;                           // W columns, then one sink per lane
+  uint32_t* bm = (uint32_t*)(dense + W + kWave);    // their bits: (W + 64) / 32 words
+  unsigned char* uni = dyn_raw + (((size_t)(W + kWave) * sizeof(V) + (size_t)((W + kWave) >> 5) * 4 + 15) & ~(size_t)15);
   SegBias<V>* sb = (SegBias<V>*)uni;
   __shared__ uint32_t pbits[2][kPassWords];
   __shared__ uint2 pword[kPassWords];  // per pass word: (start bits, set bits before the word)
@@ -1257,8 +1257,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   __shared__ int32_t s_rank[T / kWave];
   __shared__ unsigned long long s_ring[4];
   if (rp_c[A.m] > cap) return;
-  const int words = W >> 5;
-  for (int i = threadIdx.x; i < W; i += T) dense[i] = V(0);
+  const int words = (W + kWave) >> 5;
+  for (int i = threadIdx.x; i < W + kWave; i += T) dense[i] = V(0);
   for (int i = threadIdx.x; i < words; i += T) bm[i] = 0u;
   for (int i = threadIdx.x; i < 2 * kPassWords; i += T) (&pbits[0][0])[i] = 0u;
   const unsigned long long ntask = (unsigned long long)*d_ntask;
@@ -1373,32 +1373,78 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
         const int ue = (n + T - 1) / T;
         const int xb = wave_id() * kWave * ue + lane_id();
         const int cov = cover[P];
-        DenseItem<V> item[U];
-        bool ok[U];
+        // A pass runs as one of U straight-line bodies, chosen by its slot
+        // count ue.  Its slots go in two groups: a group's LDS reads, then
+        // its segment reads, then its loads are each issued for the whole
+        // group before the first result is used, and the second group is
+        // issued before the first is applied, so that the latencies of both
+        // overlap.  With no branch between them the waits count exactly.
+        auto body = [&](auto ue_c) {
+          constexpr int UE = decltype(ue_c)::value;
+          constexpr int G0 = UE <= 4 ? UE : (UE + 1) / 2;
+          DenseItem<V> item[UE];
+          bool ok[UE];
+          auto issue = [&](auto g0_c, auto g_c) {
+            constexpr int g0 = decltype(g0_c)::value, G = decltype(g_c)::value;
+            int xs[G];
+            uint2 pw[G];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          ok[u] = false;
-          if (u < ue) {
-            int x = xb + u * kWave;
-            ok[u] = x < n;
-            x = ok[u] ? x : n - 1;
-            const uint2 pw = pword[x >> 5];
-            const int c = cov + (int)pw.y + __popc(pw.x & ((2u << (x & 31)) - 1u));
-            const SegBias<V> s = sb[c];
-            const int64_t q = s.bias + t0 + x;
-            item[u] = DenseItem<V>{(uint32_t)bcol[q], bval[q], s.a};
-          }
-        }
-        // hub columns recur: the bit is tested first, so that mostly unset
-        // bits take the (same-word serialised) atomic OR
+            for (int j = 0; j < G; ++j) {
+              const int x = xb + (g0 + j) * kWave;
+              ok[g0 + j] = x < n;  // only the last slot row runs past n
+              xs[j] = x < n ? x : n - 1;
+              pw[j] = pword[xs[j] >> 5];
+            }
+            SegBias<V> sg[G];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (ok[u]) {
-            const uint32_t k = item[u].key;
-            atomicAdd(&dense[k], item[u].a * item[u].val);
-            const uint32_t bit = 1u << (k & 31);
-            if (!(bm[k >> 5] & bit)) atomicOr(&bm[k >> 5], bit);
-          }
+            for (int j = 0; j < G; ++j) {
+              const int c = cov + (int)pw[j].y + __popc(pw[j].x & ((2u << (xs[j] & 31)) - 1u));
+              sg[j] = sb[c];
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              const int64_t q = sg[j].bias + t0 + xs[j];
+              item[g0 + j] = DenseItem<V>{(uint32_t)bcol[q], bval[q], sg[j].a};
+            }
+          };
+          // hub columns recur: the bit is tested first, so that mostly unset
+          // bits take the (same-word serialised) atomic OR; a group's adds
+          // and bit reads go out before its first test.  Slots past n add
+          // into the lane's sink column W + lane, outside every chunk, so
+          // that no branch guards the adds
+          auto apply = [&](auto g0_c, auto g_c) {
+            constexpr int g0 = decltype(g0_c)::value, G = decltype(g_c)::value;
+            uint32_t kk[G], bw[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              kk[j] = ok[g0 + j] ? item[g0 + j].key : (uint32_t)(W + lane_id());
+              atomicAdd(&dense[kk[j]], item[g0 + j].a * item[g0 + j].val);
+              bw[j] = bm[kk[j] >> 5];
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              const uint32_t bit = 1u << (kk[j] & 31);
+              if (!(bw[j] & bit)) atomicOr(&bm[kk[j] >> 5], bit);
+            }
+          };
+          using I0 = std::integral_constant<int, 0>;
+          using IG0 = std::integral_constant<int, G0>;
+          using IG1 = std::integral_constant<int, UE - G0>;
+          issue(I0{}, IG0{});
+          if constexpr (UE > G0) issue(IG0{}, IG1{});
+          apply(I0{}, IG0{});
+          if constexpr (UE > G0) apply(IG0{}, IG1{});
+        };
+        static_assert(U >= 1 && U <= 8, "pass bodies for 1..8 slots");
+        switch (ue) {
+          case 1: body(std::integral_constant<int, 1>{}); break;
+          case 2: if constexpr (U >= 2) body(std::integral_constant<int, 2>{}); break;
+          case 3: if constexpr (U >= 3) body(std::integral_constant<int, 3>{}); break;
+          case 4: if constexpr (U >= 4) body(std::integral_constant<int, 4>{}); break;
+          case 5: if constexpr (U >= 5) body(std::integral_constant<int, 5>{}); break;
+          case 6: if constexpr (U >= 6) body(std::integral_constant<int, 6>{}); break;
+          case 7: if constexpr (U >= 7) body(std::integral_constant<int, 7>{}); break;
+          default: if constexpr (U >= 8) body(std::integral_constant<int, 8>{}); break;
         }
       }
       // the next window's B segment bounds (its A columns have arrived)
@@ -1413,15 +1459,19 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     }
     __syncthreads();  // every product of the morsel is in dense/bm
     // rank and emit by waves: wave w owns words [w*wpw, w*wpw + wpw) of this
-    // chunk's bitmap; its popcount goes through LDS (one barrier), then the
-    // wave walks its words two at a time, one bit per lane: set bits get
-    // their rank by ballot, so each step stores a coalesced run of C
+    // chunk's bitmap, held one word per lane (64 at a time); its popcount
+    // goes through LDS (one barrier).  The wave then takes its non-zero
+    // words two at a time from registers, one bit per lane: set bits get
+    // their rank by ballot, so each step stores a coalesced run of C.  Two
+    // steps are in flight together, and each value is read and cleared by
+    // one LDS exchange.
     const int mwords = (d.cwidth + 31) >> 5;
     const int wpw = (mwords + T / kWave - 1) / (T / kWave);
     const int wv0 = wave_id() * wpw;
     const int wv1 = (wv0 + wpw) < mwords ? (wv0 + wpw) : mwords;
-    int32_t pc = 0;
-    for (int w = wv0 + lane_id(); w < wv1; w += kWave) pc += __popc(bm[w]);
+    const uint32_t word0 = (wv0 + lane_id() < wv1) ? bm[wv0 + lane_id()] : 0u;
+    int32_t pc = __popc(word0);
+    for (int w = wv0 + kWave + lane_id(); w < wv1; w += kWave) pc += __popc(bm[w]);
     pc = wave_sum(pc);
     if (lane_id() == 0) s_rank[wave_id()] = pc;
     __syncthreads();
@@ -1435,21 +1485,44 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     if (threadIdx.x == 0 && tot != d.cnt) atomicOr(err, 4);
     const int half = lane_id() >> 5, bit = lane_id() & 31;
     const unsigned long long below = (1ull << lane_id()) - 1;
-    for (int w = wv0; w < wv1; w += 2) {
-      const int ww = w + half;
-      const uint32_t x = ww < wv1 ? bm[ww] : 0u;
-      const bool set = (x >> bit) & 1u;
-      const unsigned long long mask = __ballot(set);
-      if (set) {
-        const int32_t r = base + (int32_t)__popcll(mask & below);
-        const int u = (ww << 5) + bit;
-        if (r < d.cnt) {
-          col_c[d.off + r] = d.colbase + u;
-          val_c[d.off + r] = dense[u];
+    for (int g = wv0; g < wv1; g += kWave) {
+      const uint32_t word = g == wv0 ? word0 : ((g + lane_id() < wv1) ? bm[g + lane_id()] : 0u);
+      unsigned long long nz = __ballot(word != 0u);
+      // lowest non-zero word of the group left in nz (its lane), 64 if none
+      auto take = [&]() -> int {
+        const int i = (int)__builtin_ctzg(nz, kWave);
+        nz &= nz - 1;
+        return i;
+      };
+      auto lane_word = [&](int i) -> uint32_t {
+        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)word, i & (kWave - 1));
+        return i < kWave ? w : 0u;
+      };
+      int32_t* colp = col_c + d.off;
+      V* valp = val_c + d.off;
+      while (nz) {
+        const int i0 = take(), i1 = take(), i2 = take(), i3 = take();
+        const uint32_t w0 = lane_word(i0), w1 = lane_word(i1), w2 = lane_word(i2), w3 = lane_word(i3);
+        const uint32_t xa = half ? w1 : w0, xb = half ? w3 : w2;
+        const int ia = half ? i1 : i0, ib = half ? i3 : i2;
+        const bool seta = (xa >> bit) & 1u, setb = (xb >> bit) & 1u;
+        const unsigned long long ma = __ballot(seta), mb = __ballot(setb);
+        const int32_t ra = base + (int32_t)__popcll(ma & below);
+        const int32_t rb = base + (int32_t)__popcll(ma) + (int32_t)__popcll(mb & below);
+        base += (int32_t)(__popcll(ma) + __popcll(mb));
+        const int ua = ((g + ia) << 5) + bit, ub = ((g + ib) << 5) + bit;
+        V va = V(0), vb = V(0);
+        if (seta) va = lds_take(&dense[ua]);
+        if (setb) vb = lds_take(&dense[ub]);
+        if (seta && ra < d.cnt) {
+          colp[ra] = d.colbase + ua;
+          valp[ra] = va;
         }
-        dense[u] = V(0);
+        if (setb && rb < d.cnt) {
+          colp[rb] = d.colbase + ub;
+          valp[rb] = vb;
+        }
       }
-      base += (int32_t)__popcll(mask);
     }
     wave_lds_sync();  // the wave's reads of its words are done
     for (int w = wv0 + lane_id(); w < wv1; w += kWave) bm[w] = 0u;

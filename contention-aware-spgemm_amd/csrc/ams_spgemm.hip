@@ -257,7 +257,7 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
 // 16 bytes), then the segment table of a window.
 template <int T>
 static size_t dense_lds(int W, int vbytes) {
-  const size_t head = ((size_t)W * vbytes + (size_t)(W / 32) * 4 + 15) & ~(size_t)15;
+  const size_t head = ((size_t)(W + 64) * vbytes + (size_t)((W + 64) / 32) * 4 + 15) & ~(size_t)15;
   return head + sizeof(SegBias<double>) * T;
 }
 // Static LDS of k_numc_dense<T> (pass bitmaps, prefix counts, scan and
