@@ -105,6 +105,24 @@ __device__ __forceinline__ I block_excl_scan(I x, I* s_wave, I& total) {
   return base + incl - x;
 }
 
+// block_excl_scan without its closing barrier, for scratch that is not
+// written again before every thread has passed another barrier.
+template <int T, typename I>
+__device__ __forceinline__ I block_excl_scan_once(I x, I* s_wave, I& total) {
+  const I incl = wave_incl_scan(x);
+  if (lane_id() == kWave - 1) s_wave[wave_id()] = incl;
+  __syncthreads();
+  I base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < T / kWave; ++w) {
+    const I v = s_wave[w];
+    base += (w < wave_id()) ? v : I(0);
+    tot += v;
+  }
+  total = tot;
+  return base + incl - x;
+}
+
 template <int T, typename I>
 __device__ __forceinline__ I block_sum(I x, I* s_wave) {
   I total;

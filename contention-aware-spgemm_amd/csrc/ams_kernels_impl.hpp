@@ -1243,24 +1243,24 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
                                                   int32_t* err) {
   constexpr int U = dense_u<T>();
   constexpr int kPassWords = T * U / 32;
-  constexpr int kWpl = kPassWords / kWave;  // pass bitmap words per lane of wave 0
+  constexpr int kWpl = kPassWords / kWave;  // pass bitmap words per lane
   static_assert(kPassWords % kWave == 0, "one wave scans the pass bitmap");
   extern __shared__ __align__(16) unsigned char dyn_raw[];
   V* dense = (V*)dyn_raw;                           // W columns, then one sink per lane
   uint32_t* bm = (uint32_t*)(dense + W + kWave);    // their bits: (W + 64) / 32 words
   unsigned char* uni = dyn_raw + (((size_t)(W + kWave) * sizeof(V) + (size_t)((W + kWave) >> 5) * 4 + 15) & ~(size_t)15);
   SegBias<V>* sb = (SegBias<V>*)uni;
-  __shared__ uint32_t pbits[2][kPassWords];
-  __shared__ uint2 pword[kPassWords];  // per pass word: (start bits, set bits before the word)
-  __shared__ int32_t cover[2];
-  __shared__ int64_t s_w[T / kWave];
+  __shared__ uint32_t pbits[3][kPassWords];
+  __shared__ uint2 wpword[T / kWave][2 * U];  // per wave: (start bits, set bits before) of its pass words
+  __shared__ int32_t cover[3];
+  __shared__ int64_t s_w[2][T / kWave];
   __shared__ int32_t s_rank[T / kWave];
   __shared__ unsigned long long s_ring[4];
   if (rp_c[A.m] > cap) return;
   const int words = (W + kWave) >> 5;
   for (int i = threadIdx.x; i < W + kWave; i += T) dense[i] = V(0);
   for (int i = threadIdx.x; i < words; i += T) bm[i] = 0u;
-  for (int i = threadIdx.x; i < 2 * kPassWords; i += T) (&pbits[0][0])[i] = 0u;
+  for (int i = threadIdx.x; i < 3 * kPassWords; i += T) (&pbits[0][0])[i] = 0u;
   const unsigned long long ntask = (unsigned long long)*d_ntask;
   const V* av = (const V*)A.val;
   const int64_t ld = mB + 1;
@@ -1300,7 +1300,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     st = bp[k];
     len = (int32_t)(bp[k + 1] - st);
   }
-  unsigned ps = 0;  // pass counter: parity picks the pass bitmap
+  int pb = 0;       // pass bitmap of the next pass (three in turn)
+  unsigned wc = 0;  // window counter: parity picks the scan scratch
   for (unsigned mc = 0; t_cur < ntask; ++mc) {
     // slot (mc+3)&3 was last read at morsel mc-1 and is read next at mc+1;
     // the barriers of every morsel order both
@@ -1314,7 +1315,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
       // scan of (non-empty count << 32 | length): offsets and compact index
       int64_t tot64;
-      const int64_t ex = block_excl_scan<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
+      const int64_t ex = block_excl_scan_once<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len,
+                                                          s_w[wc++ & 1], tot64);
       const int32_t off = (int32_t)(ex & 0xffffffffll);
       const int32_t ci = (int32_t)(ex >> 32);
       const int32_t total = (int32_t)(tot64 & 0xffffffffll);
@@ -1340,39 +1342,57 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
         na = av[np + threadIdx.x];
       }
       if (len > 0) sb[ci] = SegBias<V>{st - off, a};
-      for (int32_t t0 = 0; t0 < total; t0 += T * U, ++ps) {
-        const int P = ps & 1;
+      for (int32_t t0 = 0; t0 < total; t0 += T * U) {
+        const int Bf = pb;
+        pb = pb == 2 ? 0 : pb + 1;
         const int32_t n = (total - t0) < T * U ? (total - t0) : T * U;
         if (len > 0) {
           if (off <= t0 && t0 < off + len) {
-            cover[P] = ci;
+            cover[Bf] = ci;
           } else if (off > t0 && off < t0 + n) {
             const int32_t x = off - t0;
-            atomicOr(&pbits[P][x >> 5], 1u << (x & 31));
+            atomicOr(&pbits[Bf][x >> 5], 1u << (x & 31));
           }
         }
         __syncthreads();
-        if (wave_id() == 0) {  // prefix popcounts: lane l owns words [l*WPL, l*WPL + WPL)
-          int cw[kWpl];
-          int sum = 0;
+        // the bitmap after next was last read in the previous pass, before
+        // the barrier above, and is next written after the next one
+        {
+          const int Bc = pb == 2 ? 0 : pb + 1;
+          for (int i = threadIdx.x; i < kPassWords; i += T) pbits[Bc][i] = 0u;
+        }
+        // every wave ranks the pass's start bits itself: lane l holds words
+        // [l*kWpl, l*kWpl + kWpl) and the count of set bits before them
+        uint32_t wb[kWpl];
+        int32_t wpre[kWpl];
+        {
+          int32_t sum = 0;
 #pragma unroll
           for (int q = 0; q < kWpl; ++q) {
-            cw[q] = __popc(pbits[P][lane_id() * kWpl + q]);
-            sum += cw[q];
+            wb[q] = pbits[Bf][lane_id() * kWpl + q];
+            sum += __popc(wb[q]);
           }
-          int run = wave_incl_scan(sum) - sum;
+          int32_t run = wave_incl_scan(sum) - sum;
 #pragma unroll
           for (int q = 0; q < kWpl; ++q) {
-            pword[lane_id() * kWpl + q] = make_uint2(pbits[P][lane_id() * kWpl + q], (uint32_t)run);
-            run += cw[q];
+            wpre[q] = run;
+            run += __popc(wb[q]);
           }
         }
-        // the other bitmap's last readers finished before the barrier above
-        for (int i = threadIdx.x; i < kPassWords; i += T) pbits[P ^ 1][i] = 0u;
-        __syncthreads();
         const int ue = (n + T - 1) / T;
-        const int xb = wave_id() * kWave * ue + lane_id();
-        const int cov = cover[P];
+        const int xw = wave_id() * kWave * ue;  // the wave's first position
+        const int xb = xw + lane_id();
+        const int cov = cover[Bf];
+        // the wave's own pass words [xw/32, xw/32 + 2*ue) go to its LDS row
+        {
+          const int k0 = xw >> 5;
+#pragma unroll
+          for (int q = 0; q < kWpl; ++q) {
+            const int k = lane_id() * kWpl + q - k0;
+            if (k >= 0 && k < 2 * ue) wpword[wave_id()][k] = make_uint2(wb[q], (uint32_t)wpre[q]);
+          }
+          wave_lds_sync();
+        }
         // A pass runs as one of U straight-line bodies, chosen by its slot
         // count ue.  Its slots go in two groups: a group's LDS reads, then
         // its segment reads, then its loads are each issued for the whole
@@ -1386,6 +1406,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
           bool ok[UE];
           auto issue = [&](auto g0_c, auto g_c) {
             constexpr int g0 = decltype(g0_c)::value, G = decltype(g_c)::value;
+            // no segment starts at or past n, so a position past n finds
+            // the segment of n - 1, which it then reads
             int xs[G];
             uint2 pw[G];
 #pragma unroll
@@ -1393,12 +1415,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
               const int x = xb + (g0 + j) * kWave;
               ok[g0 + j] = x < n;  // only the last slot row runs past n
               xs[j] = x < n ? x : n - 1;
-              pw[j] = pword[xs[j] >> 5];
+              pw[j] = wpword[wave_id()][((g0 + j) * kWave + lane_id()) >> 5];
             }
             SegBias<V> sg[G];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
-              const int c = cov + (int)pw[j].y + __popc(pw[j].x & ((2u << (xs[j] & 31)) - 1u));
+              const int x = xb + (g0 + j) * kWave;
+              const int c = cov + (int)pw[j].y + __popc(pw[j].x & ((2u << (x & 31)) - 1u));
               sg[j] = sb[c];
             }
 #pragma unroll

@@ -263,7 +263,7 @@ static size_t dense_lds(int W, int vbytes) {
 // Static LDS of k_numc_dense<T> (pass bitmaps, prefix counts, scan and
 // ticket scratch), bounded from above.
 template <int T>
-constexpr size_t dense_static_lds() { return (4 * T * dense_u<T>() / 32 + 2) * 4 + 12 * (T / 64) + 64; }
+constexpr size_t dense_static_lds() { return 5 * T * dense_u<T>() / 8 + T / 2 + 64; }
 
 // Dense numeric block: 512 threads while two blocks fit a CU's LDS, else
 // 1024 threads so that one block still brings 16 waves.
