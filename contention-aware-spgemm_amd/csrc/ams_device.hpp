@@ -174,9 +174,12 @@ __device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, 
 //                            that the U loads stay in flight together
 //   apply(item)              runs for the lanes whose product exists
 // All threads of the block must call it (it holds barriers).
-template <int T, int U, class Seg, class Load, class Apply>
-__device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
-                                             Apply apply) {
+// block_expand_batch hands each thread's U items of a stride to
+// apply_batch(item[U], ok[U]) at once, so that it can issue its LDS reads
+// for all of them before it uses the first.
+template <int T, int U, class Seg, class Load, class ApplyBatch>
+__device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
+                                                   ApplyBatch apply_batch) {
   using Item = decltype(load(0, int64_t(0), int64_t(0), int64_t(0)));
   int64_t base = 0;
   for (int64_t w0 = p0; w0 < p1; w0 += T) {
@@ -213,13 +216,22 @@ __device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T
         }
         item[u] = load(cur, w0 + cur, seg_bias + t, base + t);
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (ok[u]) apply(item[u]);
+      apply_batch(item, ok);
     }
     base += total;
     __syncthreads();
   }
+}
+
+template <int T, int U, class Seg, class Load, class Apply>
+__device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
+                                             Apply apply) {
+  using Item = decltype(load(0, int64_t(0), int64_t(0), int64_t(0)));
+  block_expand_batch<T, U>(p0, p1, L, seg, load, [&](Item (&item)[U], bool (&ok)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (ok[u]) apply(item[u]);
+  });
 }
 
 // Reads an LDS accumulator and leaves 0 in it, in one exchange.

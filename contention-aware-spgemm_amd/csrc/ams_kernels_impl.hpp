@@ -563,17 +563,26 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
-      block_expand<T, kUSymBitmap>(A.rp[row], A.rp[row + 1], L, seg,
-                         [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
-                         [&](int32_t col) {
-                           // hub columns recur: test the bit first, so that only
-                           // unset bits take the (same-word serialised) atomic
-                           const uint32_t u = (uint32_t)((int64_t)col - base);
-                           if (u < (uint32_t)pw) {
-                             const uint32_t bit = 1u << (u & 31);
-                             if (!(bm[u >> 5] & bit)) atomicOr(&bm[u >> 5], bit);
-                           }
-                         });
+      // hub columns recur: the bit is tested first, so that only unset bits
+      // take the (same-word serialised) atomic; a stride's bit reads all go
+      // out before the first test
+      block_expand_batch<T, kUSymBitmap>(
+          A.rp[row], A.rp[row + 1], L, seg, [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+          [&](int32_t (&col)[kUSymBitmap], bool (&ok)[kUSymBitmap]) {
+            uint32_t u[kUSymBitmap], w[kUSymBitmap];
+#pragma unroll
+            for (int j = 0; j < kUSymBitmap; ++j) {
+              u[j] = (uint32_t)((int64_t)col[j] - base);
+              const bool in = ok[j] && u[j] < (uint32_t)pw;
+              u[j] = in ? u[j] : 0u;
+              w[j] = in ? bm[u[j] >> 5] : ~0u;
+            }
+#pragma unroll
+            for (int j = 0; j < kUSymBitmap; ++j) {
+              const uint32_t bit = 1u << (u[j] & 31);
+              if (!(w[j] & bit)) atomicOr(&bm[u[j] >> 5], bit);
+            }
+          });
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {
         const int w = w0 + threadIdx.x;
@@ -1482,9 +1491,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     }
     __syncthreads();  // every product of the morsel is in dense/bm
     // rank and emit by waves: wave w owns words [w*wpw, w*wpw + wpw) of this
-    // chunk's bitmap, held one word per lane (64 at a time); its popcount
-    // goes through LDS (one barrier).  The wave then takes its non-zero
-    // words two at a time from registers, one bit per lane: set bits get
+    // chunk's bitmap, 64 words at a time; its popcount goes through LDS
+    // (one barrier).  The wave then takes its non-zero words (found by a
+    // ballot) two at a time, one bit per lane: set bits get
     // their rank by ballot, so each step stores a coalesced run of C.  Two
     // steps are in flight together, and each value is read and cleared by
     // one LDS exchange.
@@ -1517,17 +1526,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
         nz &= nz - 1;
         return i;
       };
-      auto lane_word = [&](int i) -> uint32_t {
-        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)word, i & (kWave - 1));
-        return i < kWave ? w : 0u;
-      };
       int32_t* colp = col_c + d.off;
       V* valp = val_c + d.off;
       while (nz) {
         const int i0 = take(), i1 = take(), i2 = take(), i3 = take();
-        const uint32_t w0 = lane_word(i0), w1 = lane_word(i1), w2 = lane_word(i2), w3 = lane_word(i3);
-        const uint32_t xa = half ? w1 : w0, xb = half ? w3 : w2;
         const int ia = half ? i1 : i0, ib = half ? i3 : i2;
+        const uint32_t xa = ia < kWave ? bm[g + ia] : 0u, xb = ib < kWave ? bm[g + ib] : 0u;
         const bool seta = (xa >> bit) & 1u, setb = (xb >> bit) & 1u;
         const unsigned long long ma = __ballot(seta), mb = __ballot(setb);
         const int32_t ra = base + (int32_t)__popcll(ma & below);
