@@ -140,7 +140,7 @@ class BatchedProduct:
 # its candidates, keeping the best so far (the same space as ams_autotune).
 TUNE_SPACE = [
     ("hash", [(682, 682), (1365, 1365), (2730, 2730), (1365, 2730), (2730, 5461), (5461, 5461)]),
-    ("dense_width", [4288, 8640, 12800, 17344]),
+    ("dense_width", [4288, 8448, 12800, 17344]),
     ("acc_thres", [0.0, 1.25, 2.0, 4.0]),
     ("bitmap_ratio", [0.25, 1.0, 4.0]),
     ("dense_density", [0.02, 0.05, 0.2, 1.0]),

@@ -77,8 +77,20 @@ constexpr int kUSymBitmap = 16;
 #ifndef AMS_DENSE_U1024
 #define AMS_DENSE_U1024 6
 #endif
+#ifndef AMS_DENSE_U512
+#define AMS_DENSE_U512 12
+#endif
 template <int T>
-constexpr int dense_u() { return T >= 1024 ? AMS_DENSE_U1024 : 8; }
+constexpr int dense_u() { return T >= 1024 ? AMS_DENSE_U1024 : AMS_DENSE_U512; }
+
+// f(integral_constant<int, i>) for i in [B, E), unrolled at compile time.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 template <typename V>
 struct KeyVal {
@@ -1403,18 +1415,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
           wave_lds_sync();
         }
         // A pass runs as one of U straight-line bodies, chosen by its slot
-        // count ue.  Its slots go in two groups: a group's LDS reads, then
-        // its segment reads, then its loads are each issued for the whole
-        // group before the first result is used, and the second group is
-        // issued before the first is applied, so that the latencies of both
-        // overlap.  With no branch between them the waits count exactly.
+        // count ue.  Its slots go in groups of 2 to 4: a group's LDS reads,
+        // then its segment reads, then its loads are each issued for the
+        // whole group before the first result is used, and each group is
+        // issued before the one before it is applied, so that two groups'
+        // latencies always overlap.  With no branch between them the
+        // compiler counts the waits exactly.
         auto body = [&](auto ue_c) {
           constexpr int UE = decltype(ue_c)::value;
-          constexpr int G0 = UE <= 4 ? UE : (UE + 1) / 2;
+          constexpr int NG = (UE + 3) / 4;  // groups of 3 or 4 slots (2 for 5 slots)
+          // group i: G(i) slots from g0(i), sizes as even as possible
+          auto gsize = [](int i) constexpr { return UE / NG + (i < UE % NG ? 1 : 0); };
+          auto gstart = [](int i) constexpr { return i * (UE / NG) + (i < UE % NG ? i : UE % NG); };
           DenseItem<V> item[UE];
           bool ok[UE];
-          auto issue = [&](auto g0_c, auto g_c) {
-            constexpr int g0 = decltype(g0_c)::value, G = decltype(g_c)::value;
+          auto issue = [&](auto gi_c) {
+            constexpr int g0 = gstart(decltype(gi_c)::value);
+            constexpr int G = gsize(decltype(gi_c)::value);
             // no segment starts at or past n, so a position past n finds
             // the segment of n - 1, which it then reads
             int xs[G];
@@ -1435,7 +1452,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
             }
 #pragma unroll
             for (int j = 0; j < G; ++j) {
-              const int64_t q = sg[j].bias + t0 + xs[j];
+              const int32_t tx = t0 + xs[j];  // 32-bit: the clamp above stays 32-bit too
+              const int64_t q = sg[j].bias + (int64_t)tx;
               item[g0 + j] = DenseItem<V>{(uint32_t)bcol[q], bval[q], sg[j].a};
             }
           };
@@ -1444,8 +1462,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
           // and bit reads go out before its first test.  Slots past n add
           // into the lane's sink column W + lane, outside every chunk, so
           // that no branch guards the adds
-          auto apply = [&](auto g0_c, auto g_c) {
-            constexpr int g0 = decltype(g0_c)::value, G = decltype(g_c)::value;
+          auto apply = [&](auto gi_c) {
+            constexpr int g0 = gstart(decltype(gi_c)::value);
+            constexpr int G = gsize(decltype(gi_c)::value);
             uint32_t kk[G], bw[G];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
@@ -1459,25 +1478,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
               if (!(bw[j] & bit)) atomicOr(&bm[kk[j] >> 5], bit);
             }
           };
-          using I0 = std::integral_constant<int, 0>;
-          using IG0 = std::integral_constant<int, G0>;
-          using IG1 = std::integral_constant<int, UE - G0>;
-          issue(I0{}, IG0{});
-          if constexpr (UE > G0) issue(IG0{}, IG1{});
-          apply(I0{}, IG0{});
-          if constexpr (UE > G0) apply(IG0{}, IG1{});
+          issue(std::integral_constant<int, 0>{});
+          static_for<1, NG>([&](auto gi_c) {
+            issue(gi_c);
+            apply(std::integral_constant<int, decltype(gi_c)::value - 1>{});
+          });
+          apply(std::integral_constant<int, NG - 1>{});
         };
-        static_assert(U >= 1 && U <= 8, "pass bodies for 1..8 slots");
-        switch (ue) {
-          case 1: body(std::integral_constant<int, 1>{}); break;
-          case 2: if constexpr (U >= 2) body(std::integral_constant<int, 2>{}); break;
-          case 3: if constexpr (U >= 3) body(std::integral_constant<int, 3>{}); break;
-          case 4: if constexpr (U >= 4) body(std::integral_constant<int, 4>{}); break;
-          case 5: if constexpr (U >= 5) body(std::integral_constant<int, 5>{}); break;
-          case 6: if constexpr (U >= 6) body(std::integral_constant<int, 6>{}); break;
-          case 7: if constexpr (U >= 7) body(std::integral_constant<int, 7>{}); break;
-          default: if constexpr (U >= 8) body(std::integral_constant<int, 8>{}); break;
-        }
+        static_assert(U >= 1 && U <= 16, "pass bodies for 1..16 slots");
+        static_for<1, U + 1>([&](auto uc) {
+          if (ue == decltype(uc)::value) body(uc);
+        });
       }
       // the next window's B segment bounds (its A columns have arrived)
       st = 0;

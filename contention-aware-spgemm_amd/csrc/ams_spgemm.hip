@@ -753,7 +753,7 @@ void ams_default_params(ams_params_t* p) {
   p->num_hash_max = 2730;
   p->esc_max = 4096;
   p->acc_thres = 2.0;
-  p->dense_width = 8640;
+  p->dense_width = 8448;
   p->bitmap_bits = 1 << 20;
   p->bitmap_ratio = 1.0;
   p->dense_density = 0.05;
@@ -1179,7 +1179,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
     return r;
   }
   const int hash_pairs[][2] = {{682, 682}, {1365, 1365}, {2730, 2730}, {1365, 2730}, {2730, 5461}, {5461, 5461}};
-  const int widths[] = {4288, 8640, 12800, 17344, 24576};
+  const int widths[] = {4288, 8448, 12800, 17344, 24576};
   const double thres[] = {0.0, 1.25, 2.0, 4.0};
   const double ratios[] = {0.25, 1.0, 4.0};
   const double dens[] = {0.02, 0.05, 0.2, 1.0};

@@ -80,10 +80,10 @@ def test_rectangular_unsorted(gpu_ctx, route):
     assert_matches_oracle(c, a, b, exact=False)
 
 
-@pytest.mark.parametrize("width", [8640, 14848])
+@pytest.mark.parametrize("width", [8448, 14848])
 def test_wide_chunks(gpu_ctx, width):
     """Chunks wide enough for the 1024-thread dense kernel (14848 columns)
-    next to the default 8640-column chunks, on a 16384-column problem."""
+    next to the default 8448-column chunks, on a 16384-column problem."""
     a = gen.rmat_symmetric(14, 2, seed=77)
     print(f"[parity] rmat_s14_ef2 dense_width={width}", flush=True)
     c = run(gpu_ctx, a, a, dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_density=0.0,
