@@ -385,6 +385,72 @@ __device__ __forceinline__ void block_bitonic_sort(K* s, int n) {
   }
 }
 
+// Lane i ^ m's value (m < 64) for 32- or 64-bit keys.
+template <typename K>
+__device__ __forceinline__ K shfl_xor_key(K x, int m) {
+  if constexpr (sizeof(K) == 8) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m, kWave);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m, kWave);
+    return (K)(((uint64_t)hi << 32) | lo);
+  } else {
+    return (K)__shfl_xor((int)x, m, kWave);
+  }
+}
+
+// One compare-exchange stage of stride J < IPT inside every thread's
+// registers (element e = base + r sits in v[r]).
+template <int J, int IPT, typename K>
+__device__ __forceinline__ void reg_cx_stage(K (&v)[IPT], int base, int k) {
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    if ((r & J) == 0) {
+      const bool asc = ((base + r) & k) == 0;
+      const K a = v[r], b = v[r + J];
+      const bool sw = (a > b) == asc;
+      v[r] = sw ? b : a;
+      v[r + J] = sw ? a : b;
+    }
+  }
+}
+
+// Ascending bitonic sort of T*IPT keys held IPT per thread, thread-contiguous
+// (element e = threadIdx.x*IPT + r in v[r]).  Strides below IPT compare
+// registers, strides below 64*IPT exchange with lane (tid ^ stride/IPT) of
+// the same wave by shuffles, and only the longer ones go through LDS (s,
+// T*IPT keys) with barriers: a 2048-key sort by 256 threads has 3 such
+// stages out of 66.
+template <int T, int IPT, typename K>
+__device__ __forceinline__ void reg_bitonic_sort(K (&v)[IPT], K* s) {
+  static_assert(IPT == 1 || IPT == 2 || IPT == 4 || IPT == 8, "1, 2, 4 or 8 keys per thread");
+  constexpr int N = T * IPT;
+  const int base = threadIdx.x * IPT;
+  for (int k = 2; k <= N; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < IPT) {
+        if constexpr (IPT > 1) { if (j == 1) reg_cx_stage<1, IPT>(v, base, k); }
+        if constexpr (IPT > 2) { if (j == 2) reg_cx_stage<2, IPT>(v, base, k); }
+        if constexpr (IPT > 4) { if (j == 4) reg_cx_stage<4, IPT>(v, base, k); }
+      } else {
+        const bool far = j >= kWave * IPT;
+        if (far) {
+#pragma unroll
+          for (int r = 0; r < IPT; ++r) s[base + r] = v[r];
+          __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+          const int e = base + r;
+          const K o = far ? s[e ^ j] : shfl_xor_key(v[r], j / IPT);
+          const bool asc = (e & k) == 0, lower = (e & j) == 0;
+          const K lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+          v[r] = lower == asc ? lo : hi;
+        }
+        if (far) __syncthreads();
+      }
+    }
+  }
+}
+
 // Sorts perm[0..n) = 0..n-1 by the composite (key[i], i) ascending: a
 // counting pass over nb (power of two, >= n) order-preserving buckets of
 // the key range [kmin, kmin + span), a scatter, and an insertion sort of

@@ -783,7 +783,7 @@ template <int E, int T, typename V>
 __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* list,
                                                 const int32_t* counts, int bin, const int64_t* row_flp,
                                                 const int64_t* rp_c, int32_t* col_c, V* val_c,
-                                                int64_t cap, int esc_sort, int32_t* err) {
+                                                int64_t cap, int esc_sort, int key32, int32_t* err) {
   static_assert(E <= 65536, "ranks are uint16");
   constexpr int IPT = (E + T - 1) / T;
   __shared__ __align__(16) unsigned char region[E * 10];
@@ -823,20 +823,32 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
                        });
     const int nb = (int)next_pow2_i64(f);
     if (esc_sort == 0) {
-      uint32_t kr[IPT];
+      // (column, rank) keys sorted in registers: packed in 32 bits as
+      // column << log2(E) | rank when the columns allow, else 64 bits
+      constexpr int RB = __builtin_ctz(E);
+      static_assert(T * IPT == E, "the bin fills the threads' registers");
+      const int i0 = threadIdx.x * IPT;
+      if (key32) {
+        uint32_t kr[IPT];
 #pragma unroll
-      for (int u = 0; u < IPT; ++u) {
-        const int i = threadIdx.x + u * T;
-        kr[u] = i < f ? key[i] : 0u;
+        for (int r = 0; r < IPT; ++r)
+          kr[r] = i0 + r < f ? ((key[i0 + r] << RB) | (uint32_t)(i0 + r)) : 0xffffffffu;
+        __syncthreads();
+        reg_bitonic_sort<T, IPT>(kr, (uint32_t*)sk);
+#pragma unroll
+        for (int r = 0; r < IPT; ++r)
+          sk[i0 + r] = (((uint64_t)(kr[r] >> RB)) << 32) | (uint64_t)(kr[r] & (uint32_t)(E - 1));
+      } else {
+        uint64_t kr[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; ++r)
+          kr[r] = i0 + r < f ? (((uint64_t)key[i0 + r] << 32) | (uint64_t)(i0 + r)) : ~0ull;
+        __syncthreads();
+        reg_bitonic_sort<T, IPT>(kr, sk);
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) sk[i0 + r] = kr[r];
       }
       __syncthreads();
-#pragma unroll
-      for (int u = 0; u < IPT; ++u) {
-        const int i = threadIdx.x + u * T;
-        if (i < nb) sk[i] = i < f ? (((uint64_t)kr[u] << 32) | (uint64_t)i) : ~0ull;
-      }
-      __syncthreads();
-      block_bitonic_sort<T, (E / 2 + T - 1) / T, uint64_t>(sk, nb);
     } else {
       uint32_t kmin = 0xffffffffu, kmax = 0u;
       for (int i = threadIdx.x; i < f; i += T) {

@@ -553,9 +553,12 @@ static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const in
   occupancy(k_numc_esc<E, T, V>, T, 0, &occ);
   const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
   Ctrl* c = dctrl(ctx);
+  // (column, rank) fits 32 bits as column << log2(E) | rank below 2^(32 - log2(E)) columns
+  const int key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(E))) ? 1 : 0;
   hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
                      (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
-                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
+                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, key32,
+                     &c->err);
 }
 
 template <int T, typename V>
