@@ -385,15 +385,50 @@ __device__ __forceinline__ void block_bitonic_sort(K* s, int n) {
   }
 }
 
-// Lane i ^ m's value (m < 64) for 32- or 64-bit keys.
-template <typename K>
-__device__ __forceinline__ K shfl_xor_key(K x, int m) {
+// Lane i ^ M's value (M < 64, a power of two).  M = 1, 2 and 8 are one DPP
+// move (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:8), M = 4 two DPP
+// row shifts and a select; only 16 and 32 cross rows through ds_bpermute.
+template <int M>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+  if constexpr (M == 1) {
+    return dpp_move<0xb1, 0xf>(x);
+  } else if constexpr (M == 2) {
+    return dpp_move<0x4e, 0xf>(x);
+  } else if constexpr (M == 4) {
+    const uint32_t up = dpp_move<0x104, 0xf>(x);    // row_shl:4, lane i + 4
+    const uint32_t down = dpp_move<0x114, 0xf>(x);  // row_shr:4, lane i - 4
+    return (lane_id() & 4) ? down : up;
+  } else if constexpr (M == 8) {
+    return dpp_move<0x128, 0xf>(x);
+  } else {
+    return (uint32_t)__shfl_xor((int)x, M, kWave);
+  }
+}
+
+template <int M, typename K>
+__device__ __forceinline__ K lane_xor_key(K x) {
   if constexpr (sizeof(K) == 8) {
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m, kWave);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m, kWave);
+    const uint32_t lo = lane_xor<M>((uint32_t)x);
+    const uint32_t hi = lane_xor<M>((uint32_t)(x >> 32));
     return (K)(((uint64_t)hi << 32) | lo);
   } else {
-    return (K)__shfl_xor((int)x, m, kWave);
+    return (K)lane_xor<M>((uint32_t)x);
+  }
+}
+
+// Compare-exchange of every register with lane tid ^ M (stride j = M*IPT,
+// merge size k): all exchanges of the stage go out before any compare.
+template <int M, int IPT, typename K>
+__device__ __forceinline__ void lane_cx_stage(K (&v)[IPT], int base, int k, int j) {
+  K o[IPT];
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) o[r] = lane_xor_key<M>(v[r]);
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    const int e = base + r;
+    const bool asc = (e & k) == 0, lower = (e & j) == 0;
+    const K lo = v[r] < o[r] ? v[r] : o[r], hi = v[r] < o[r] ? o[r] : v[r];
+    v[r] = lower == asc ? lo : hi;
   }
 }
 
@@ -431,21 +466,28 @@ __device__ __forceinline__ void reg_bitonic_sort(K (&v)[IPT], K* s) {
         if constexpr (IPT > 2) { if (j == 2) reg_cx_stage<2, IPT>(v, base, k); }
         if constexpr (IPT > 4) { if (j == 4) reg_cx_stage<4, IPT>(v, base, k); }
       } else {
-        const bool far = j >= kWave * IPT;
-        if (far) {
+        switch (j / IPT) {
+          case 1: lane_cx_stage<1, IPT>(v, base, k, j); break;
+          case 2: lane_cx_stage<2, IPT>(v, base, k, j); break;
+          case 4: lane_cx_stage<4, IPT>(v, base, k, j); break;
+          case 8: lane_cx_stage<8, IPT>(v, base, k, j); break;
+          case 16: lane_cx_stage<16, IPT>(v, base, k, j); break;
+          case 32: lane_cx_stage<32, IPT>(v, base, k, j); break;
+          default: {  // another wave's registers: through LDS
 #pragma unroll
-          for (int r = 0; r < IPT; ++r) s[base + r] = v[r];
-          __syncthreads();
-        }
+            for (int r = 0; r < IPT; ++r) s[base + r] = v[r];
+            __syncthreads();
 #pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-          const int e = base + r;
-          const K o = far ? s[e ^ j] : shfl_xor_key(v[r], j / IPT);
-          const bool asc = (e & k) == 0, lower = (e & j) == 0;
-          const K lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
-          v[r] = lower == asc ? lo : hi;
+            for (int r = 0; r < IPT; ++r) {
+              const int e = base + r;
+              const K o = s[e ^ j];
+              const bool asc = (e & k) == 0, lower = (e & j) == 0;
+              const K lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+              v[r] = lower == asc ? lo : hi;
+            }
+            __syncthreads();
+          }
         }
-        if (far) __syncthreads();
       }
     }
   }
