@@ -544,6 +544,127 @@ __global__ __launch_bounds__(T) void k_symb_bitmap_wave(Csr A, Csr B, Probs pr, 
   }
 }
 
+// Heavy rows of the bitmap bin.  A row with more than kPiece products
+// (kPiece = max(kMinPiece, P / kMaxHeavy)) is split into q <= 64 pieces of
+// equal A-nonzero ranges.  Blocks OR each piece's LDS bitmap into the row's
+// global bitmap (gbm, words of the row's column range), and
+// k_heavy_finish counts it per chunk.  Since every heavy row has more than
+// P / kMaxHeavy products there are at most kMaxHeavy of them, and at most
+// 2 * kMaxHeavy pieces.
+constexpr int kMaxHeavy = 2048;
+constexpr int kMaxPieces = 2 * kMaxHeavy;
+constexpr int64_t kMinPiece = 1 << 20;
+constexpr int kMaxPiecesPerRow = 64;
+
+struct HeavyPiece {
+  int32_t h;    // index in the bitmap bin
+  int32_t hid;  // heavy row slot (its gbm row)
+  int32_t i, q; // piece i of q
+};
+
+__global__ __launch_bounds__(256) void k_heavy_plan(const Probs pr, const int64_t* row_flp, const int32_t* list,
+                                                    const int32_t* counts, int bin, const unsigned long long* flops,
+                                                    uint8_t* hflag, int32_t* hrow, HeavyPiece* pieces,
+                                                    int32_t* nheavy, int32_t* npieces) {
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const int64_t per = (int64_t)(*flops / (unsigned long long)kMaxHeavy);
+  const int64_t kPiece = per > kMinPiece ? per : kMinPiece;
+  for (int32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < cnt; h += gridDim.x * blockDim.x) {
+    const int32_t row = list[start + h];
+    const int64_t f = row_flp[row];
+    if (f <= kPiece) {
+      hflag[h] = 0;
+      continue;
+    }
+    int64_t q64 = (f + kPiece - 1) / kPiece;
+    const int32_t q = (int32_t)(q64 < kMaxPiecesPerRow ? q64 : kMaxPiecesPerRow);
+    // the counts above bound both reservations; should a total of products
+    // ever undercount, a row that does not fit stays whole and the slots it
+    // took are marked empty (h = -1)
+    const int32_t hid = atomicAdd(nheavy, 1);
+    if (hid >= kMaxHeavy) {
+      hflag[h] = 0;
+      continue;
+    }
+    const int32_t base = atomicAdd(npieces, q);
+    const bool fits = base + q <= kMaxPieces;
+    hflag[h] = fits ? 1 : 0;
+    hrow[hid] = fits ? h : -1;
+    for (int32_t i = 0; i < q && base + i < kMaxPieces; ++i)
+      pieces[base + i] = fits ? HeavyPiece{h, hid, i, q} : HeavyPiece{-1, -1, 0, 1};
+  }
+}
+
+// Per heavy row: population of every W-column chunk of its global bitmap
+// into coff (exclusive prefix, total at index R) and rownz; the bitmap is
+// cleared for the next call.
+template <int T>
+__global__ __launch_bounds__(T) void k_heavy_finish(const Probs pr, const int32_t* list, const int32_t* counts,
+                                                    int bin, const int32_t* hrow, const int32_t* nheavy,
+                                                    uint32_t* gbm, int64_t gwords, int64_t* rownz, int32_t* coff,
+                                                    int R, int W) {
+  __shared__ int64_t s_w[T / kWave];
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const int32_t nh = *nheavy < kMaxHeavy ? *nheavy : kMaxHeavy;
+  const int wchunk = W >> 5;
+  for (int32_t hid = blockIdx.x; hid < nh; hid += gridDim.x) {
+    const int32_t h = hrow[hid];
+    if (h < 0) continue;
+    const int32_t row = list[start + h];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    const int64_t words = (width + 31) >> 5;
+    uint32_t* g = gbm + (int64_t)hid * gwords;
+    int32_t* crow = coff + (int64_t)h * (R + 1);
+    const int nch = (int)((width + W - 1) / W);
+    int32_t running = 0;
+    for (int c = 0; c < nch; ++c) {
+      const int64_t w0 = (int64_t)c * wchunk;
+      const int64_t w1 = (w0 + wchunk) < words ? (w0 + wchunk) : words;
+      int64_t pc = 0;
+      for (int64_t w = w0 + threadIdx.x; w < w1; w += T) {
+        pc += __popc(g[w]);
+        g[w] = 0u;
+      }
+      const int64_t tot = block_sum<T, int64_t>(pc, s_w);
+      if (threadIdx.x == 0) crow[c] = running;
+      running += (int32_t)tot;
+    }
+    if (threadIdx.x == 0) {
+      for (int c = nch; c <= R; ++c) crow[c] = running;
+      rownz[row] = running;
+    }
+  }
+}
+
+// Sets the bits [base, base + pw) of the columns of the products of A
+// nonzeros [p0, p1) in the LDS bitmap bm.  Hub columns recur: the bit is
+// tested first, so that only unset bits take the (same-word serialised)
+// atomic; a stride's bit reads all go out before the first test.
+template <int T>
+__device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, const SegB& seg,
+                                                   const Csr& B, uint32_t* bm, int64_t base, int32_t pw) {
+  block_expand_batch<T, kUSymBitmap>(
+      p0, p1, L, seg, [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+      [&](int32_t (&col)[kUSymBitmap], bool (&ok)[kUSymBitmap]) {
+        uint32_t u[kUSymBitmap], w[kUSymBitmap];
+#pragma unroll
+        for (int j = 0; j < kUSymBitmap; ++j) {
+          u[j] = (uint32_t)((int64_t)col[j] - base);
+          const bool in = ok[j] && u[j] < (uint32_t)pw;
+          u[j] = in ? u[j] : 0u;
+          w[j] = in ? bm[u[j] >> 5] : ~0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kUSymBitmap; ++j) {
+          const uint32_t bit = 1u << (u[j] & 31);
+          if (!(w[j] & bit)) atomicOr(&bm[u[j] >> 5], bit);
+        }
+      });
+}
+
 // LDS-bitmap symbolic for wide rows, one row per block.  The row's output
 // columns [0, width) are covered in passes of Wb bits; the population of
 // every W-column chunk goes to coff (exclusive prefix, total at index R).
@@ -551,7 +672,9 @@ __global__ __launch_bounds__(T) void k_symb_bitmap_wave(Csr A, Csr B, Probs pr, 
 template <int T>
 __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
                                                    const int32_t* list, const int32_t* counts, int bin,
-                                                   int64_t* rownz, int32_t* coff, int R, int W, int Wb) {
+                                                   int64_t* rownz, int32_t* coff, int R, int W, int Wb,
+                                                   const uint8_t* hflag, const HeavyPiece* pieces,
+                                                   const int32_t* npieces, uint32_t* gbm, int64_t gwords) {
   extern __shared__ uint32_t dyn[];
   uint32_t* bm = dyn;
   int32_t* ccount = (int32_t*)(dyn + (Wb >> 5));
@@ -560,11 +683,38 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
   bin_range(counts, bin, start, cnt);
   const SegB seg{A.col, B.rp};
   const int wchunk = W >> 5;
+  // the pieces of heavy rows first (they are the longest work items): each
+  // ORs its bitmap into the row's global bitmap, word by non-zero word
+  const int32_t np = hflag ? (*npieces < kMaxPieces ? *npieces : kMaxPieces) : 0;
+  for (int32_t pi = blockIdx.x; pi < np; pi += gridDim.x) {
+    const HeavyPiece hp = pieces[pi];
+    if (hp.h < 0) continue;
+    const int32_t row = list[start + hp.h];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    const int64_t p0 = A.rp[row], deg = A.rp[row + 1] - p0;
+    const int64_t q0 = p0 + deg * hp.i / hp.q, q1 = p0 + deg * (hp.i + 1) / hp.q;
+    uint32_t* g = gbm + (int64_t)hp.hid * gwords;
+    for (int64_t c0 = 0; c0 < width; c0 += Wb) {
+      const int32_t pw = (int32_t)((width - c0) < Wb ? (width - c0) : Wb);
+      const int32_t words = (pw + 31) >> 5;
+      for (int w = threadIdx.x; w < words; w += T) bm[w] = 0u;
+      __syncthreads();
+      const int64_t base = lo + c0;
+      symb_bitmap_expand<T>(q0, q1, L, seg, B, bm, base, pw);
+      for (int w = threadIdx.x; w < words; w += T) {
+        const uint32_t x = bm[w];
+        if (x) atomicOr(&g[(c0 >> 5) + w], x);
+      }
+      __syncthreads();
+    }
+  }
   for (int32_t h = blockIdx.x; h < cnt; h += gridDim.x) {
     const int32_t row = list[start + h];
     int64_t lo, width;
     prob_cols(pr, row, lo, width);
     if (width <= kWaveBitmapBits && row_flp[row] <= kWaveBitmapFlops) continue;  // k_symb_bitmap_wave's rows
+    if (hflag && hflag[h]) continue;  // split into pieces above
     int32_t running = 0;
     int32_t* crow = coff + (int64_t)h * (R + 1);
     for (int64_t c0 = 0; c0 < width; c0 += Wb) {
@@ -575,26 +725,7 @@ __global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
-      // hub columns recur: the bit is tested first, so that only unset bits
-      // take the (same-word serialised) atomic; a stride's bit reads all go
-      // out before the first test
-      block_expand_batch<T, kUSymBitmap>(
-          A.rp[row], A.rp[row + 1], L, seg, [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
-          [&](int32_t (&col)[kUSymBitmap], bool (&ok)[kUSymBitmap]) {
-            uint32_t u[kUSymBitmap], w[kUSymBitmap];
-#pragma unroll
-            for (int j = 0; j < kUSymBitmap; ++j) {
-              u[j] = (uint32_t)((int64_t)col[j] - base);
-              const bool in = ok[j] && u[j] < (uint32_t)pw;
-              u[j] = in ? u[j] : 0u;
-              w[j] = in ? bm[u[j] >> 5] : ~0u;
-            }
-#pragma unroll
-            for (int j = 0; j < kUSymBitmap; ++j) {
-              const uint32_t bit = 1u << (u[j] & 31);
-              if (!(w[j] & bit)) atomicOr(&bm[u[j] >> 5], bit);
-            }
-          });
+      symb_bitmap_expand<T>(A.rp[row], A.rp[row + 1], L, seg, B, bm, base, pw);
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {
         const int w = w0 + threadIdx.x;

@@ -45,7 +45,9 @@ struct Ctrl {
   int32_t nlong;  // long B rows listed for the wave-per-row blocked copy
   int32_t pad1;
   unsigned long long sums[16];
-  int64_t pad2[5];
+  int32_t nheavy;   // heavy rows of the bitmap bin split into pieces
+  int32_t npieces;
+  int64_t pad2[4];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 
@@ -68,6 +70,8 @@ struct ams_ctx {
   std::string err;
   Buf row_flp, sbin, nbin, lsym, lnum, ctrl, coff, bptr, bcol, bval, bperm, tsum;
   Buf pool_rp, pool_col, pool_val, probs, prow, pbrow, scratch_a, dflag, dtask, ccnt, lrows;
+  Buf hflag, hrow, hpiece, gbm;  // heavy bitmap rows: flags, slots, pieces, global bitmaps
+  size_t gbm_zero = 0;           // bytes of gbm known to be zero
   bool bblk_ready = false;  // blocked B of the current plan already built
   Ctrl* h_ctrl = nullptr;       // pinned mirror of the control block
   int64_t* h_pin = nullptr;     // pinned scratch
@@ -251,6 +255,27 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
   if (wb < W) wb = W;
   Wb = (int)wb;
   (void)dtype;
+}
+
+// Words of a heavy row's global bitmap (the widest problem's columns), or 0
+// when the kMaxHeavy bitmaps would take more than 512 MiB (no splitting).
+static int64_t heavy_words(const Probs& pr) {
+  const int64_t gwords = (std::max<int64_t>(pr.max_width, 1) + 31) / 32;
+  return (size_t)gwords * 4 * kMaxHeavy <= ((size_t)512 << 20) ? gwords : 0;
+}
+
+// Buffers of the heavy-row split for a bitmap bin of up to H rows; the
+// global bitmaps start zeroed and k_heavy_finish leaves them zero.
+static int grow_heavy(ams_ctx_t* ctx, const Probs& pr, int64_t H) {
+  CHK(grow(ctx, ctx->hflag, (size_t)std::max<int64_t>(H, 1)));
+  CHK(grow(ctx, ctx->hrow, sizeof(int32_t) * kMaxHeavy));
+  CHK(grow(ctx, ctx->hpiece, sizeof(HeavyPiece) * kMaxPieces));
+  CHK(grow(ctx, ctx->gbm, (size_t)heavy_words(pr) * 4 * kMaxHeavy));
+  if (ctx->gbm_zero < ctx->gbm.cap) {
+    CHK(fill_i32(ctx, (int32_t*)ctx->gbm.p, (int64_t)(ctx->gbm.cap / 4), 0));
+    ctx->gbm_zero = ctx->gbm.cap;
+  }
+  return AMS_OK;
 }
 
 // Dynamic LDS of k_numc_dense<T>: dense values, column bitmap (padded to
@@ -452,13 +477,39 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
         return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
       HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<kSymBitmapT>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+      // heavy rows (more than max(2^20, P/2048) products) are split into
+      // pieces when their global bitmaps fit 512 MiB
+      const int64_t gwords = heavy_words(pr);
+      const bool split = gwords > 0;
+      const uint8_t* hflag = nullptr;
+      if (split) {
+        CHK(grow_heavy(ctx, pr, H));
+        const int gp = (int)std::max<int64_t>(1, std::min<int64_t>((H + 255) / 256, (int64_t)ctx->num_cu * 8));
+        hipLaunchKernelGGL(k_heavy_plan, dim3(gp), dim3(256), 0, ctx->stream, pr, (const int64_t*)ctx->row_flp.p,
+                           (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP,
+                           (const unsigned long long*)&c->flops, (uint8_t*)ctx->hflag.p, (int32_t*)ctx->hrow.p,
+                           (HeavyPiece*)ctx->hpiece.p, &c->nheavy, &c->npieces);
+        HIPCHK(hipGetLastError());
+        hflag = (const uint8_t*)ctx->hflag.p;
+      }
       int occ;
       occupancy(k_symb_bitmap<kSymBitmapT>, kSymBitmapT, dyn, &occ);
       const int g = grid_for(ctx, occ, H, ctx->plan_counts);
       hipLaunchKernelGGL(k_symb_bitmap<kSymBitmapT>, dim3(g), dim3(kSymBitmapT), dyn, ctx->stream, A, B, pr,
                          (const int64_t*)ctx->row_flp.p, (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym,
-                         (int)SB_BITMAP, rownz, (int32_t*)ctx->coff.p, R, W, Wb);
+                         (int)SB_BITMAP, rownz, (int32_t*)ctx->coff.p, R, W, Wb, hflag,
+                         (const HeavyPiece*)ctx->hpiece.p, (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p,
+                         gwords);
       HIPCHK(hipGetLastError());
+      if (split) {
+        constexpr int TF = 256;
+        const int gf = std::min(kMaxHeavy, ctx->num_cu * 2);
+        hipLaunchKernelGGL(k_heavy_finish<TF>, dim3(gf), dim3(TF), 0, ctx->stream, pr, (const int32_t*)ctx->lsym.p,
+                           (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const int32_t*)ctx->hrow.p,
+                           (const int32_t*)&c->nheavy, (uint32_t*)ctx->gbm.p, gwords, rownz,
+                           (int32_t*)ctx->coff.p, R, W);
+        HIPCHK(hipGetLastError());
+      }
       if (pr.max_width <= kWaveBitmapBits) {  // light rows of narrow problems: a wave each
         constexpr int TW = 256;
         int occw;
@@ -796,7 +847,8 @@ int ams_ctx_destroy(ams_ctx_t* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   Buf* bufs[] = {&ctx->row_flp, &ctx->sbin, &ctx->nbin, &ctx->lsym, &ctx->lnum, &ctx->ctrl, &ctx->coff,
                  &ctx->bptr, &ctx->bcol, &ctx->bval, &ctx->bperm, &ctx->tsum, &ctx->pool_rp, &ctx->pool_col,
-                 &ctx->pool_val, &ctx->probs, &ctx->scratch_a, &ctx->dflag, &ctx->dtask, &ctx->ccnt};
+                 &ctx->pool_val, &ctx->probs, &ctx->scratch_a, &ctx->dflag, &ctx->dtask, &ctx->ccnt,
+                 &ctx->prow, &ctx->pbrow, &ctx->lrows, &ctx->hflag, &ctx->hrow, &ctx->hpiece, &ctx->gbm};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : ctx->recs) {
@@ -1091,6 +1143,7 @@ int ams_batched_plan(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, i
   CHK(grow(ctx, ctx->dtask, sizeof(DenseTask) * std::max<int64_t>(m, 1) * R));
   const int64_t nmax = std::max<int64_t>(std::max<int64_t>(m + 1, (int64_t)R * ld), std::max<int64_t>(m, 1) * R + 1);
   CHK(grow(ctx, ctx->tsum, sizeof(int64_t) * ((nmax + kScanTile - 1) / kScanTile)));
+  if (heavy_words(pr) > 0) CHK(grow_heavy(ctx, pr, m));
   ctx->batch_pr = pr;
   ctx->batch_m = m;
   ctx->batch_mB = b->num_row;

@@ -92,6 +92,25 @@ def test_wide_chunks(gpu_ctx, width):
     assert_matches_oracle(c, a, a, exact=True)
 
 
+@pytest.mark.parametrize("bitmap_bits", [1 << 20, 128])
+def test_heavy_rows_split(gpu_ctx, bitmap_bits):
+    """C = A^T A of a join-shaped incidence whose hub columns give rows of
+    more than max(2^20, P/2048) products: those rows are counted in pieces
+    by several blocks (OR-ed global bitmaps), in one bitmap pass or in
+    several (bitmap_bits 128 on 300 columns)."""
+    inc = gen.join_incidence(num_row=60000, num_col=300, mu=3.5, sigma=0.5, cap=100, seed=5)
+    at = inc.transpose()
+    deg = np.diff(np.asarray(inc.row_ptrs)).astype(np.int64)
+    rows = np.repeat(np.arange(at.num_row), np.diff(np.asarray(at.row_ptrs)))
+    row_p = np.bincount(rows, weights=deg[np.asarray(at.cols)], minlength=at.num_row)
+    assert row_p.max() > max(1 << 20, row_p.sum() / 2048), "the case must have a heavy row"
+    print(f"[parity] join 60000x300 heavy rows {(row_p > max(1 << 20, row_p.sum() / 2048)).sum()} "
+          f"bitmap_bits={bitmap_bits}", flush=True)
+    c = run(gpu_ctx, at, inc, dict(bitmap_bits=bitmap_bits, dense_width=128 if bitmap_bits < 1024 else 8448))
+    assert c.sorted_rows()
+    assert_matches_oracle(c, at, inc, exact=True)
+
+
 def test_edge_shapes(gpu_ctx):
     z = Csr(5, 7, np.zeros(6, np.int64), np.zeros(0, np.int32), np.zeros(0))
     b = random_csr(7, 9, 0.5, seed=4)
