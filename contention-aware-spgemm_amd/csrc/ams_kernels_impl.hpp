@@ -72,7 +72,10 @@ struct SegB {
 
 // Products in flight per lane in the expansion (loads issued before the
 // LDS updates they feed), per kernel.
-constexpr int kUSymBitmap = 16;
+#ifndef AMS_SYMB_U
+#define AMS_SYMB_U 16
+#endif
+constexpr int kUSymBitmap = AMS_SYMB_U;
 // products per thread and pass of the dense numeric kernel (register bound)
 #ifndef AMS_DENSE_U1024
 #define AMS_DENSE_U1024 6
@@ -669,8 +672,13 @@ __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, Expan
 // columns [0, width) are covered in passes of Wb bits; the population of
 // every W-column chunk goes to coff (exclusive prefix, total at index R).
 // Dynamic LDS: Wb/32 words of bitmap + Wb/W chunk counters.
+// Eight waves per SIMD (64 VGPRs): two 1024-thread blocks share a CU's LDS,
+// twice the waves of 512-thread blocks, 15% faster on G500 s19 ef32.
+#ifndef AMS_SYMB_WPE
+#define AMS_SYMB_WPE 8
+#endif
 template <int T>
-__global__ __launch_bounds__(T) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE))) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
                                                    const int32_t* list, const int32_t* counts, int bin,
                                                    int64_t* rownz, int32_t* coff, int R, int W, int Wb,
                                                    const uint8_t* hflag, const HeavyPiece* pieces,

@@ -51,7 +51,10 @@ struct Ctrl {
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 
-constexpr int kSymBitmapT = 512;
+#ifndef AMS_SYMB_T
+#define AMS_SYMB_T 1024
+#endif
+constexpr int kSymBitmapT = AMS_SYMB_T;
 constexpr int kDenseT = 512;
 constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 
