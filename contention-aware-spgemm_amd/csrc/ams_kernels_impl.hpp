@@ -125,6 +125,16 @@ struct RankGather {
   V val;
 };
 
+// A product of the expansion with its A value gathered alongside (no LDS
+// staging of the window's A values).
+template <typename V>
+struct RankGatherA {
+  int32_t rank;
+  int32_t key;
+  V a;
+  V val;
+};
+
 // ------------------------------------------------------------- row flops
 
 // Row flops in groups of 64 rows per wave: rows of at most kFlpShort
@@ -932,7 +942,6 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
   uint64_t* sk = (uint64_t*)region;
   __shared__ V vals[E];
   __shared__ ExpandLds<T> L;
-  __shared__ V aval[T];
   __shared__ int64_t s_w[T / kWave];
   __shared__ uint32_t s_min[T / kWave], s_max[T / kWave];
   __shared__ int32_t s_flag;
@@ -951,14 +960,13 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
                          const int32_t k = A.col[p];
                          st = B.rp[k];
                          len = (int32_t)(B.rp[k + 1] - st);
-                         aval[threadIdx.x] = av[p];
                        },
-                       [&](int j, int64_t, int64_t q, int64_t t) {
-                         return RankGather<V>{(int32_t)t, B.col[q], j, bv[q]};
+                       [&](int, int64_t p, int64_t q, int64_t t) {
+                         return RankGatherA<V>{(int32_t)t, B.col[q], av[p], bv[q]};
                        },
-                       [&](RankGather<V> r) {
+                       [&](RankGatherA<V> r) {
                          key[r.rank] = (uint32_t)r.key;
-                         vals[r.rank] = aval[r.j] * r.val;
+                         vals[r.rank] = r.a * r.val;
                        });
     const int nb = (int)next_pow2_i64(f);
     if (esc_sort == 0) {
