@@ -928,18 +928,23 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
 // permutation (esc_sort = 1); both share one LDS region laid out as
 //   bucket:  cnt int32[E] | key uint32[E] | perm uint16[E]
 //   bitonic: sk uint64[E] (over cnt and key)   | key copy kept in regs
-template <int E, int T, typename V>
+// K32: the columns fit column << log2(E) | rank in 32 bits and the bitonic
+// sort is used; the sorted keys then go back into key[] and the LDS region
+// is 4E bytes instead of 10E (five 2048-product blocks per CU instead of
+// four).
+template <int E, int T, typename V, bool K32>
 __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* list,
                                                 const int32_t* counts, int bin, const int64_t* row_flp,
                                                 const int64_t* rp_c, int32_t* col_c, V* val_c,
-                                                int64_t cap, int esc_sort, int key32, int32_t* err) {
+                                                int64_t cap, int esc_sort, int32_t* err) {
   static_assert(E <= 65536, "ranks are uint16");
   constexpr int IPT = (E + T - 1) / T;
-  __shared__ __align__(16) unsigned char region[E * 10];
+  constexpr int RB = __builtin_ctz(E);
+  __shared__ __align__(16) unsigned char region[K32 ? E * 4 : E * 10];
   int32_t* cnt = (int32_t*)region;
-  uint32_t* key = (uint32_t*)(region + 4 * E);
-  uint16_t* perm = (uint16_t*)(region + 8 * E);
-  uint64_t* sk = (uint64_t*)region;
+  uint32_t* key = (uint32_t*)(region + (K32 ? 0 : 4 * E));
+  uint16_t* perm = (uint16_t*)(region + 8 * E);  // bucket sort only (not K32)
+  uint64_t* sk = (uint64_t*)region;             // 64-bit sorted keys (not K32)
   __shared__ V vals[E];
   __shared__ ExpandLds<T> L;
   __shared__ int64_t s_w[T / kWave];
@@ -969,22 +974,20 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
                          vals[r.rank] = r.a * r.val;
                        });
     const int nb = (int)next_pow2_i64(f);
-    if (esc_sort == 0) {
+    if (K32 || esc_sort == 0) {
       // (column, rank) keys sorted in registers: packed in 32 bits as
       // column << log2(E) | rank when the columns allow, else 64 bits
-      constexpr int RB = __builtin_ctz(E);
       static_assert(T * IPT == E, "the bin fills the threads' registers");
       const int i0 = threadIdx.x * IPT;
-      if (key32) {
+      if constexpr (K32) {
         uint32_t kr[IPT];
 #pragma unroll
         for (int r = 0; r < IPT; ++r)
           kr[r] = i0 + r < f ? ((key[i0 + r] << RB) | (uint32_t)(i0 + r)) : 0xffffffffu;
         __syncthreads();
-        reg_bitonic_sort<T, IPT>(kr, (uint32_t*)sk);
+        reg_bitonic_sort<T, IPT>(kr, key);
 #pragma unroll
-        for (int r = 0; r < IPT; ++r)
-          sk[i0 + r] = (((uint64_t)(kr[r] >> RB)) << 32) | (uint64_t)(kr[r] & (uint32_t)(E - 1));
+        for (int r = 0; r < IPT; ++r) key[i0 + r] = kr[r];
       } else {
         uint64_t kr[IPT];
 #pragma unroll
@@ -996,7 +999,7 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
         for (int r = 0; r < IPT; ++r) sk[i0 + r] = kr[r];
       }
       __syncthreads();
-    } else {
+    } else if constexpr (!K32) {
       uint32_t kmin = 0xffffffffu, kmax = 0u;
       for (int i = threadIdx.x; i < f; i += T) {
         kmin = min(kmin, key[i]);
@@ -1015,8 +1018,14 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
       block_sort_perm<T>(key, perm, cnt, f, nb, kmin, (uint64_t)(kmax - kmin) + 1, &s_flag);
     }
     // sorted element e: column and rank
-    auto col_at = [&](int e) -> uint32_t { return esc_sort == 0 ? (uint32_t)(sk[e] >> 32) : key[perm[e]]; };
-    auto rank_at = [&](int e) -> uint32_t { return esc_sort == 0 ? (uint32_t)sk[e] : (uint32_t)perm[e]; };
+    auto col_at = [&](int e) -> uint32_t {
+      if constexpr (K32) return key[e] >> RB;
+      else return esc_sort == 0 ? (uint32_t)(sk[e] >> 32) : key[perm[e]];
+    };
+    auto rank_at = [&](int e) -> uint32_t {
+      if constexpr (K32) return key[e] & (uint32_t)(E - 1);
+      else return esc_sort == 0 ? (uint32_t)sk[e] : (uint32_t)perm[e];
+    };
     const int32_t ipt = nb >= T ? nb / T : 1;
     const int32_t e0 = threadIdx.x * ipt;
     const int32_t e1 = (e0 + ipt) < f ? (e0 + ipt) : f;

@@ -597,22 +597,28 @@ static void launch_numc_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, const i
                      &c->err);
 }
 
+template <int E, typename V, bool K32>
+static void launch_numc_esc_k(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
+                              V* val_c, int64_t cap, int bin, int64_t cnt) {
+  constexpr int T = hash_threads<E>();
+  int occ;
+  occupancy(k_numc_esc<E, T, V, K32>, T, 0, &occ);
+  const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+  Ctrl* c = dctrl(ctx);
+  hipLaunchKernelGGL((k_numc_esc<E, T, V, K32>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                     (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
+                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
+}
+
 template <int E, typename V>
 static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
                             V* val_c, int64_t cap, int bin) {
-  constexpr int T = hash_threads<E>();
   const int64_t cnt = ctx->plan_counts ? ctx->cnt_num[bin] : A.m;
   if (ctx->plan_counts && cnt == 0) return;
-  int occ;
-  occupancy(k_numc_esc<E, T, V>, T, 0, &occ);
-  const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
-  Ctrl* c = dctrl(ctx);
   // (column, rank) fits 32 bits as column << log2(E) | rank below 2^(32 - log2(E)) columns
-  const int key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(E))) ? 1 : 0;
-  hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
-                     (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
-                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, key32,
-                     &c->err);
+  const bool key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(E))) && ctx->prm.esc_sort == 0;
+  if (key32) launch_numc_esc_k<E, V, true>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
+  else launch_numc_esc_k<E, V, false>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
 }
 
 template <int T, typename V>

@@ -111,6 +111,22 @@ def test_heavy_rows_split(gpu_ctx, bitmap_bits):
     assert_matches_oracle(c, at, inc, exact=True)
 
 
+def test_esc_wide_columns(gpu_ctx):
+    """ESC rows over 2^26 + 7 columns: (column, rank) no longer packs into 32
+    bits for any bin, so the 64-bit key sort runs (random FP64 values)."""
+    rng = np.random.default_rng(17)
+    n = (1 << 26) + 7
+    a = random_csr(200, 300, 0.05, seed=18)
+    rows = np.repeat(np.arange(300), 20)
+    cols = np.sort(rng.choice(n, size=(300, 20), replace=False if n > 20 else True), axis=1).reshape(-1)
+    rp = np.arange(0, 301 * 20, 20, dtype=np.int64)
+    b = Csr(300, n, rp, cols.astype(np.int32), rng.uniform(-1, 1, len(cols)))
+    print("[parity] esc over 2^26+7 columns", flush=True)
+    c = run(gpu_ctx, a, b, dict(mode=2, esc_max=4096))
+    assert c.sorted_rows()
+    assert_matches_oracle(c, a, b, exact=False)
+
+
 def test_edge_shapes(gpu_ctx):
     z = Csr(5, 7, np.zeros(6, np.int64), np.zeros(0, np.int32), np.zeros(0))
     b = random_csr(7, 9, 0.5, seed=4)
