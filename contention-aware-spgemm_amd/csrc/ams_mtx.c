@@ -248,6 +248,9 @@ void mtx2csr(csr_t *csr, const char *filename, numa_cfg_t *numa_cfg) {
 	dealloc_memory(trpl, sizeof(triple_t) * (size_t)num_nnz);
 }
 
+/* src/mm/mm_utils.c:730-732: the drivers' name for mtx2csr. */
+void raw2csr(csr_t *csr, const char *filename, numa_cfg_t *numa_cfg) { mtx2csr(csr, filename, numa_cfg); }
+
 void trpl_ones(triple_t *trpl, my_cnt_t num_nnz) {
 	for (my_cnt_t i = 0; i < num_nnz; ++i) trpl[i].rating = 1.0;
 }

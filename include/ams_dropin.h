@@ -117,6 +117,8 @@ void trpl2mtx(const char *filename, triple_t *trpl, my_cnt_t num_row, my_cnt_t n
 void trpl2csr(csr_t *csr, triple_t *trpl, const my_cnt_t num_row, const my_cnt_t num_col, const my_cnt_t num_nnz,
 	numa_cfg_t *numa_cfg);
 void mtx2csr(csr_t *csr, const char *filename, numa_cfg_t *numa_cfg);
+/* src/mm/mm_utils.c:730-732: what the drivers call (src/ams_hybaccspgemm.c:294-299) */
+void raw2csr(csr_t *csr, const char *filename, numa_cfg_t *numa_cfg);
 void trpl_ones(triple_t *trpl, my_cnt_t num_nnz);
 void trpl_transpose(triple_t *trpl, my_cnt_t *num_row, my_cnt_t *num_col, my_cnt_t num_nnz);
 void txt2trpl(triple_t **trpl, my_cnt_t *num_row, my_cnt_t *num_col, my_cnt_t *num_nnz, const char *filename,

@@ -122,3 +122,21 @@ def test_rmat_ones_pair(tmp_path):
 def test_missing_file_raises(tmp_path):
     with pytest.raises(FileNotFoundError):
         mtx.read_mtx(os.path.join(tmp_path, "absent.mtx"))
+
+
+def test_raw2csr_is_mtx2csr(tmp_path):
+    """raw2csr, the name the reference's drivers call (src/mm/mm_utils.c:
+    730-732), reads the same CSR as mtx2csr."""
+    import ctypes as C
+    lib = mtx._lib()
+    lib.raw2csr.argtypes = [C.POINTER(mtx.CsrT), C.c_char_p, C.c_void_p]
+    lib.raw2csr.restype = None
+    p = write(tmp_path, "sym.mtx", SYM)
+    c = mtx.CsrT()
+    lib.raw2csr(C.byref(c), os.fsencode(p), None)
+    try:
+        got = (c.num_row, c.num_col, [c.row_ptrs[i] for i in range(c.num_row + 1)],
+               [c.cols[i] for i in range(c.num_nnz)], [c.rats[i] for i in range(c.num_nnz)])
+    finally:
+        lib.free_csr(C.byref(c))
+    assert got == as_tuple(mtx.read_mtx(p))
