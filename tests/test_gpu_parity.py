@@ -162,6 +162,39 @@ def test_edge_shapes(gpu_ctx):
         assert_matches_oracle(c, a, a, exact=True)
 
 
+def with_duplicates(m: Csr, seed: int, shuffle: bool) -> Csr:
+    """m with about a third of its entries repeated inside their rows (the
+    reference keeps such repeats: its reader does not merge them)."""
+    rng = np.random.default_rng(seed)
+    rp, cols, vals = [0], [], []
+    for i in range(m.num_row):
+        s, e = int(m.row_ptrs[i]), int(m.row_ptrs[i + 1])
+        c, v = list(m.cols[s:e]), list(m.rats[s:e])
+        for j in range(e - s):
+            if rng.random() < 0.35:
+                c.append(m.cols[s + j])
+                v.append(rng.uniform(-1, 1) if m.rats.dtype == np.float64 and not np.all(m.rats == 1) else 1.0)
+        order = rng.permutation(len(c)) if shuffle else np.argsort(c, kind="stable")
+        cols += [c[k] for k in order]
+        vals += [v[k] for k in order]
+        rp.append(len(cols))
+    return Csr(m.num_row, m.num_col, np.array(rp, np.int64), np.array(cols, np.int32), np.array(vals, np.float64))
+
+
+@pytest.mark.parametrize("route", ["default", "hash_only", "esc_only", "all_dense", "big_tables"])
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_duplicate_entries(gpu_ctx, route, shuffle):
+    """Repeated column entries inside rows of A and B: every copy adds its
+    products, and C has each column once."""
+    a1 = with_duplicates(gen.rmat_symmetric(10, 4, seed=21), 22, shuffle)
+    c = run(gpu_ctx, a1, a1, ROUTES[route])
+    assert c.sorted_rows()
+    assert_matches_oracle(c, a1, a1, exact=True)
+    a2 = with_duplicates(random_csr(150, 150, 0.05, seed=23), 24, shuffle)
+    c = run(gpu_ctx, a2, a2, ROUTES[route])
+    assert_matches_oracle(c, a2, a2, exact=False)
+
+
 def test_fp32(gpu_ctx):
     a = gen.rmat_symmetric(10, 8, seed=9, values="normal", val_dtype=np.float32)
     for route in ("default", "all_dense", "esc_only"):
