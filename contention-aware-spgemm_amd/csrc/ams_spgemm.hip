@@ -503,8 +503,11 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
                          (int)SB_BITMAP, rownz, (int32_t*)ctx->coff.p, R, W, Wb, hflag,
                          (const HeavyPiece*)ctx->hpiece.p, (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p,
                          gwords);
-      HIPCHK(hipGetLastError());
       if (split) {
+        // k_heavy_finish clears what the pieces set; should either launch
+        // fail, the global bitmaps are refilled with zeros on the next call
+        ctx->gbm_zero = 0;
+        HIPCHK(hipGetLastError());
         constexpr int TF = 256;
         const int gf = std::min(kMaxHeavy, ctx->num_cu * 2);
         hipLaunchKernelGGL(k_heavy_finish<TF>, dim3(gf), dim3(TF), 0, ctx->stream, pr, (const int32_t*)ctx->lsym.p,
@@ -512,7 +515,9 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
                            (const int32_t*)&c->nheavy, (uint32_t*)ctx->gbm.p, gwords, rownz,
                            (int32_t*)ctx->coff.p, R, W);
         HIPCHK(hipGetLastError());
+        ctx->gbm_zero = ctx->gbm.cap;
       }
+      HIPCHK(hipGetLastError());
       if (pr.max_width <= kWaveBitmapBits) {  // light rows of narrow problems: a wave each
         constexpr int TW = 256;
         int occw;
