@@ -988,6 +988,57 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
         reg_bitonic_sort<T, IPT>(kr, key);
 #pragma unroll
         for (int r = 0; r < IPT; ++r) key[i0 + r] = kr[r];
+        __syncthreads();
+        // compress from the registers: thread t owns sorted elements
+        // [t*IPT, t*IPT + IPT); LDS is read only for the previous thread's
+        // last key and for a run that goes on into the next threads
+        const uint32_t pcol = (i0 > 0 && i0 <= f) ? (key[i0 - 1] >> RB) : 0u;
+        uint32_t hd = 0;
+        int64_t heads = 0;
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+          const bool head = i0 + r < f &&
+                            (r == 0 ? (i0 == 0 || (kr[0] >> RB) != pcol) : (kr[r] >> RB) != (kr[r - 1] >> RB));
+          hd |= (uint32_t)head << r;
+          heads += head;
+        }
+        int64_t tot;
+        int64_t rank = block_excl_scan<T, int64_t>(heads, s_w, tot);
+        if (threadIdx.x == 0 && tot != z) atomicOr(err, 2);
+        bool open = false;
+        uint32_t rc = 0;
+        V sum = V(0);
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+          if (i0 + r < f) {
+            const V x = vals[kr[r] & (uint32_t)(E - 1)];
+            if ((hd >> r) & 1u) {
+              if (open && rank < z) {
+                col_c[off + rank] = (int32_t)rc;
+                val_c[off + rank] = sum;
+              }
+              rank += open;
+              open = true;
+              rc = kr[r] >> RB;
+              sum = x;
+            } else if (open) {
+              sum += x;
+            }
+          }
+        }
+        if (open) {
+          for (int32_t g = i0 + IPT; g < f; ++g) {
+            const uint32_t kg = key[g];
+            if ((kg >> RB) != rc) break;
+            sum += vals[kg & (uint32_t)(E - 1)];
+          }
+          if (rank < z) {
+            col_c[off + rank] = (int32_t)rc;
+            val_c[off + rank] = sum;
+          }
+        }
+        __syncthreads();
+        continue;
       } else {
         uint64_t kr[IPT];
 #pragma unroll

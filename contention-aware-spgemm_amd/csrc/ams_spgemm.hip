@@ -55,6 +55,21 @@ static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 #define AMS_SYMB_T 1024
 #endif
 constexpr int kSymBitmapT = AMS_SYMB_T;
+// Batches of narrow problems (a bitmap pass of at most 64 Ki bits, 8 KiB of
+// LDS) count their wide rows in 256-thread blocks: eight rows per CU in
+// flight instead of two, for the many rows of a few thousand products that
+// do not fill a 1024-thread block (the 512-product batch of R-MAT scales
+// 8..14: 83.5 -> 77.3 ms).  A single problem keeps the 1024-thread blocks:
+// with few such rows the longest row's latency decides (G500 s16 ef16
+// 9.0 ms against 9.4 ms with 256 threads).
+#ifndef AMS_SYMB_NARROW_T
+#define AMS_SYMB_NARROW_T 256
+#endif
+#ifndef AMS_SYMB_NARROW_BITS
+#define AMS_SYMB_NARROW_BITS 65536
+#endif
+constexpr int kSymBitmapNarrowT = AMS_SYMB_NARROW_T;
+constexpr int kSymNarrowBits = AMS_SYMB_NARROW_BITS;
 constexpr int kDenseT = 512;
 constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 
@@ -379,6 +394,25 @@ static void launch_symb_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t
                      (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, bin, rownz, &c->err);
 }
 
+// The bitmap bin's wide rows (and heavy-row pieces, when hflag is set), one
+// row per T-thread block with dyn bytes of bitmap and chunk counters.
+template <int T>
+static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
+                              int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords) {
+  if (dyn + sizeof(ExpandLds<T>) > ctx->lds_max)
+    return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
+  HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+  Ctrl* c = dctrl(ctx);
+  int occ;
+  occupancy(k_symb_bitmap<T>, T, dyn, &occ);
+  const int g = grid_for(ctx, occ, H, ctx->plan_counts);
+  hipLaunchKernelGGL(k_symb_bitmap<T>, dim3(g), dim3(T), dyn, ctx->stream, A, B, pr, (const int64_t*)ctx->row_flp.p,
+                     (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
+                     (int32_t*)ctx->coff.p, R, W, Wb, hflag, (const HeavyPiece*)ctx->hpiece.p,
+                     (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p, gwords);
+  return AMS_OK;
+}
+
 static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype,
                         const Probs& pr, int64_t* rp_c, bool sync, int64_t* nnz) {
   const Csr A = to_csr(a), B = to_csr(b);
@@ -476,10 +510,6 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     Phase ph(ctx, AMS_PH_SYMB_BITMAP, sync);
     if (!ctx->plan_counts || H > 0) {
       const size_t dyn = (size_t)(Wb / 32) * 4 + (size_t)(Wb / W + 1) * 4;
-      if (dyn + sizeof(ExpandLds<kSymBitmapT>) > ctx->lds_max)
-        return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
-      HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<kSymBitmapT>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
       // heavy rows (more than max(2^20, P/2048) products) are split into
       // pieces when their global bitmaps fit 512 MiB
       const int64_t gwords = heavy_words(pr);
@@ -495,14 +525,10 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
         HIPCHK(hipGetLastError());
         hflag = (const uint8_t*)ctx->hflag.p;
       }
-      int occ;
-      occupancy(k_symb_bitmap<kSymBitmapT>, kSymBitmapT, dyn, &occ);
-      const int g = grid_for(ctx, occ, H, ctx->plan_counts);
-      hipLaunchKernelGGL(k_symb_bitmap<kSymBitmapT>, dim3(g), dim3(kSymBitmapT), dyn, ctx->stream, A, B, pr,
-                         (const int64_t*)ctx->row_flp.p, (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym,
-                         (int)SB_BITMAP, rownz, (int32_t*)ctx->coff.p, R, W, Wb, hflag,
-                         (const HeavyPiece*)ctx->hpiece.p, (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p,
-                         gwords);
+      if (pr.n > 1 && Wb <= kSymNarrowBits)
+        CHK(launch_symb_bitmap<kSymBitmapNarrowT>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords));
+      else
+        CHK(launch_symb_bitmap<kSymBitmapT>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords));
       if (split) {
         // k_heavy_finish clears what the pieces set; should either launch
         // fail, the global bitmaps are refilled with zeros on the next call

@@ -26,8 +26,9 @@ def small_batch(count, seed):
 @pytest.mark.parametrize("params", [{}, dict(sym_hash_max=16, num_hash_max=16, bitmap_ratio=0.0,
                                               dense_width=64, bitmap_bits=256, dense_density=0.0),
                                     dict(sym_blocked=1, sym_hash_max=16, num_hash_max=16, bitmap_ratio=0.0,
-                                         dense_width=64, bitmap_bits=256, dense_density=0.0)],
-                         ids=["default", "all_dense", "blocked_sym"])
+                                         dense_width=64, bitmap_bits=256, dense_density=0.0),
+                                    dict(sym_hash_max=1, bitmap_ratio=0.0)],
+                         ids=["default", "all_dense", "blocked_sym", "bitmap_rows"])
 def test_batched_graph_replays(gpu_ctx, params):
     import torch
     from ams_hip.batched import BatchedProduct

@@ -111,6 +111,29 @@ def test_heavy_rows_split(gpu_ctx, bitmap_bits):
     assert_matches_oracle(c, at, inc, exact=True)
 
 
+def test_wide_heavy_rows(gpu_ctx):
+    """Rows of more than 2^20 products over 150000 columns: bitmap passes of
+    more than 2^17 bits (the 1024-thread symbolic blocks) with the heavy
+    rows split into pieces, next to light rows of the same problem."""
+    rng = np.random.default_rng(31)
+    n, kb, nb = 150000, 900, 3000
+    cols = np.sort(rng.integers(0, n, size=(nb, kb)), axis=1)
+    keep = np.ones_like(cols, dtype=bool)
+    keep[:, 1:] = cols[:, 1:] != cols[:, :-1]
+    rp = np.concatenate([[0], np.cumsum(keep.sum(axis=1))]).astype(np.int64)
+    b = Csr(nb, n, rp, cols[keep].astype(np.int32), np.ones(int(rp[-1])))
+    deg = np.diff(rp)
+    arows = [np.sort(rng.choice(nb, size=k, replace=False)) for k in (1500, 1400, 40, 3, 700)]
+    a_rp = np.concatenate([[0], np.cumsum([len(r) for r in arows])]).astype(np.int64)
+    a = Csr(len(arows), nb, a_rp, np.concatenate(arows).astype(np.int32), np.ones(int(a_rp[-1])))
+    row_p = np.array([deg[r].sum() for r in arows])
+    assert row_p[:2].min() > (1 << 20), "two rows must be heavy"
+    print(f"[parity] wide heavy rows: products {row_p.tolist()}", flush=True)
+    c = run(gpu_ctx, a, b, dict(sym_hash_max=1, bitmap_ratio=0.0))
+    assert c.sorted_rows()
+    assert_matches_oracle(c, a, b, exact=True)
+
+
 def test_esc_wide_columns(gpu_ctx):
     """ESC rows over 2^26 + 7 columns: (column, rank) no longer packs into 32
     bits for any bin, so the 64-bit key sort runs (random FP64 values)."""
