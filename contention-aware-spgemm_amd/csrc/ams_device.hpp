@@ -454,43 +454,43 @@ __device__ __forceinline__ void reg_cx_stage(K (&v)[IPT], int base, int k) {
 // the same wave by shuffles, and only the longer ones go through LDS (s,
 // T*IPT keys) with barriers: a 2048-key sort by 256 threads has 3 such
 // stages out of 66.
+// Stage (merge size KS, stride J) of the bitonic network, with both known at
+// compile time: the stride picks registers, lanes or LDS, and the direction
+// masks fold to constants.
+template <int T, int IPT, int KS, int J, typename K>
+__device__ __forceinline__ void bitonic_stage(K (&v)[IPT], K* s, int base) {
+  if constexpr (J < IPT) {
+    reg_cx_stage<J, IPT>(v, base, KS);
+  } else if constexpr (J / IPT < kWave) {
+    lane_cx_stage<J / IPT, IPT>(v, base, KS, J);
+  } else {  // another wave's registers: through LDS
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) s[base + r] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      const int e = base + r;
+      const K o = s[e ^ J];
+      const bool asc = (e & KS) == 0, lower = (e & J) == 0;
+      const K lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+      v[r] = lower == asc ? lo : hi;
+    }
+    __syncthreads();
+  }
+  if constexpr (J > 1) bitonic_stage<T, IPT, KS, J / 2>(v, s, base);
+}
+
+template <int T, int IPT, int KS, typename K>
+__device__ __forceinline__ void bitonic_levels(K (&v)[IPT], K* s, int base) {
+  bitonic_stage<T, IPT, KS, KS / 2>(v, s, base);
+  if constexpr (KS < T * IPT) bitonic_levels<T, IPT, KS * 2>(v, s, base);
+}
+
 template <int T, int IPT, typename K>
 __device__ __forceinline__ void reg_bitonic_sort(K (&v)[IPT], K* s) {
   static_assert(IPT == 1 || IPT == 2 || IPT == 4 || IPT == 8, "1, 2, 4 or 8 keys per thread");
-  constexpr int N = T * IPT;
-  const int base = threadIdx.x * IPT;
-  for (int k = 2; k <= N; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j < IPT) {
-        if constexpr (IPT > 1) { if (j == 1) reg_cx_stage<1, IPT>(v, base, k); }
-        if constexpr (IPT > 2) { if (j == 2) reg_cx_stage<2, IPT>(v, base, k); }
-        if constexpr (IPT > 4) { if (j == 4) reg_cx_stage<4, IPT>(v, base, k); }
-      } else {
-        switch (j / IPT) {
-          case 1: lane_cx_stage<1, IPT>(v, base, k, j); break;
-          case 2: lane_cx_stage<2, IPT>(v, base, k, j); break;
-          case 4: lane_cx_stage<4, IPT>(v, base, k, j); break;
-          case 8: lane_cx_stage<8, IPT>(v, base, k, j); break;
-          case 16: lane_cx_stage<16, IPT>(v, base, k, j); break;
-          case 32: lane_cx_stage<32, IPT>(v, base, k, j); break;
-          default: {  // another wave's registers: through LDS
-#pragma unroll
-            for (int r = 0; r < IPT; ++r) s[base + r] = v[r];
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < IPT; ++r) {
-              const int e = base + r;
-              const K o = s[e ^ j];
-              const bool asc = (e & k) == 0, lower = (e & j) == 0;
-              const K lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
-              v[r] = lower == asc ? lo : hi;
-            }
-            __syncthreads();
-          }
-        }
-      }
-    }
-  }
+  static_assert(T * IPT >= 2, "at least two keys");
+  bitonic_levels<T, IPT, 2>(v, s, (int)threadIdx.x * IPT);
 }
 
 // Sorts perm[0..n) = 0..n-1 by the composite (key[i], i) ascending: a
