@@ -130,11 +130,12 @@ __device__ __forceinline__ I block_sum(I x, I* s_wave) {
   return total;
 }
 
-// LDS scratch of one block-level expansion window.
-template <int T>
+// LDS scratch of one block-level expansion window of NW <= T nonzeros.
+template <int T, int NW = T>
 struct ExpandLds {
-  int32_t off[T];     // exclusive prefix of segment lengths in the window
-  int64_t start[T];   // first B position of each segment
+  static_assert(NW <= T && NW % kWave == 0, "window of whole waves, at most the block");
+  int32_t off[NW];    // exclusive prefix of segment lengths in the window
+  int64_t start[NW];  // first B position of each segment
   int64_t wtmp[T / kWave];
 };
 
@@ -158,8 +159,9 @@ __device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, 
 }
 
 // Enumerates the products of A nonzeros [p0, p1): nonzero p contributes the
-// B positions [start(p), start(p) + len(p)).  Windows of T nonzeros are
-// flattened with a block scan.  In every T*U-wide stride of the window,
+// B positions [start(p), start(p) + len(p)).  Windows of NW nonzeros (the
+// block size T unless the kernel saves LDS with fewer) are flattened with a
+// block scan.  In every T*U-wide stride of the window,
 // wave w owns the 64*U consecutive products starting at w*64*U and lane l
 // takes products l, l+64, ... of them, so consecutive products of one B row
 // go to consecutive lanes and B reads coalesce whatever the segment lengths
@@ -167,7 +169,7 @@ __device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, 
 // thread keeps the segment of its last product and seeks forward from it.
 // The U products' global reads (load) are issued before their LDS updates
 // (apply), giving every lane U independent loads in flight.
-//   seg(p, start&, len&)     segment of nonzero p (p < p1), on thread p - w0
+//   seg(p, start&, len&)     segment of nonzero p (p < p1), on thread p - w0 < NW
 //   load(j, p, q, t) -> item j = index in window, q = B position,
 //                            t = rank of the product in A-row order; it must
 //                            only gather (no arithmetic on loaded values), so
@@ -177,23 +179,26 @@ __device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, 
 // block_expand_batch hands each thread's U items of a stride to
 // apply_batch(item[U], ok[U]) at once, so that it can issue its LDS reads
 // for all of them before it uses the first.
-template <int T, int U, class Seg, class Load, class ApplyBatch>
-__device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
+template <int T, int U, int NW, class Seg, class Load, class ApplyBatch>
+__device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, ExpandLds<T, NW>& L, Seg seg, Load load,
                                                    ApplyBatch apply_batch) {
   using Item = decltype(load(0, int64_t(0), int64_t(0), int64_t(0)));
   int64_t base = 0;
-  for (int64_t w0 = p0; w0 < p1; w0 += T) {
+  const bool loader = NW == T || (int)threadIdx.x < NW;
+  for (int64_t w0 = p0; w0 < p1; w0 += NW) {
     const int64_t p = w0 + threadIdx.x;
     int64_t st = 0;
     int32_t len = 0;
-    if (p < p1) seg(p, st, len);
+    if (loader && p < p1) seg(p, st, len);
     int64_t total64;
     const int64_t ex = block_excl_scan<T, int64_t>((int64_t)len, L.wtmp, total64);
-    L.off[threadIdx.x] = (int32_t)ex;
-    L.start[threadIdx.x] = st;
+    if (loader) {
+      L.off[threadIdx.x] = (int32_t)ex;
+      L.start[threadIdx.x] = st;
+    }
     __syncthreads();
     const int32_t total = (int32_t)total64;
-    const int nvalid = (int)((p1 - w0) < T ? (p1 - w0) : T);
+    const int nvalid = (int)((p1 - w0) < NW ? (p1 - w0) : NW);
     // the lane's current segment stays in registers: [.., seg_end) and
     // B position = seg_bias + t; LDS is searched only when t leaves it
     int cur = 0;
@@ -223,8 +228,8 @@ __device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, Expan
   }
 }
 
-template <int T, int U, class Seg, class Load, class Apply>
-__device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Load load,
+template <int T, int U, int NW, class Seg, class Load, class Apply>
+__device__ __forceinline__ void block_expand(int64_t p0, int64_t p1, ExpandLds<T, NW>& L, Seg seg, Load load,
                                              Apply apply) {
   using Item = decltype(load(0, int64_t(0), int64_t(0), int64_t(0)));
   block_expand_batch<T, U>(p0, p1, L, seg, load, [&](Item (&item)[U], bool (&ok)[U]) {

@@ -932,6 +932,12 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
 // sort is used; the sorted keys then go back into key[] and the LDS region
 // is 4E bytes instead of 10E (five 2048-product blocks per CU instead of
 // four).
+// ESC expansion windows of at most 128 A nonzeros: for E = 2048 the block
+// then needs 26.2 KB of LDS instead of 27.7 KB, six blocks per CU instead
+// of five.
+template <int T>
+constexpr int kEscWin = T < 128 ? T : 128;
+
 template <int E, int T, typename V, bool K32>
 __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* list,
                                                 const int32_t* counts, int bin, const int64_t* row_flp,
@@ -946,7 +952,7 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
   uint16_t* perm = (uint16_t*)(region + 8 * E);  // bucket sort only (not K32)
   uint64_t* sk = (uint64_t*)region;             // 64-bit sorted keys (not K32)
   __shared__ V vals[E];
-  __shared__ ExpandLds<T> L;
+  __shared__ ExpandLds<T, kEscWin<T>> L;  // rows of this bin have few A nonzeros
   __shared__ int64_t s_w[T / kWave];
   __shared__ uint32_t s_min[T / kWave], s_max[T / kWave];
   __shared__ int32_t s_flag;

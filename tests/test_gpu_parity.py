@@ -134,6 +134,21 @@ def test_wide_heavy_rows(gpu_ctx):
     assert_matches_oracle(c, a, b, exact=True)
 
 
+@pytest.mark.parametrize("route", ["default", "esc_only"])
+@pytest.mark.parametrize("b_density", [0.0005, 0.002, 0.0055])
+def test_esc_long_a_rows(gpu_ctx, route, b_density):
+    """ESC rows with 260+ A nonzeros over short B rows: several expansion
+    windows per row (64 A nonzeros in the 512-product bin, 128 in the 2048-
+    and 4096-product bins, whose blocks have 256 and 512 threads)."""
+    a = random_csr(24, 3000, 0.1, seed=41)
+    b = random_csr(3000, 2000, b_density, seed=42)
+    assert np.diff(a.row_ptrs).min() > 128
+    print(f"[parity] esc long A rows route={route} b_density={b_density}", flush=True)
+    c = run(gpu_ctx, a, b, ROUTES[route])
+    assert c.sorted_rows()
+    assert_matches_oracle(c, a, b, exact=False)
+
+
 def test_esc_wide_columns(gpu_ctx):
     """ESC rows over 2^26 + 7 columns: (column, rank) no longer packs into 32
     bits for any bin, so the 64-bit key sort runs (random FP64 values)."""
