@@ -1414,6 +1414,10 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, Probs pr, int W, cons
 }
 
 constexpr unsigned long long kGrab = 8;
+// Emit groups whose non-zero bitmap words average fewer set bits than this
+// take one set bit per lane; denser groups take a word per half-wave, one
+// bit per lane (G500 s19 ef32: 336.6 -> 319.9 ms; 4: 328, 16: 326 ms).
+constexpr int kSparseEmitBits = 8;
 
 // A morsel descriptor held as one dword per lane (lanes 0..7 of every
 // wave): it stays a vector load, so waits on LDS results never also wait
@@ -1728,11 +1732,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     __syncthreads();  // every product of the morsel is in dense/bm
     // rank and emit by waves: wave w owns words [w*wpw, w*wpw + wpw) of this
     // chunk's bitmap, 64 words at a time; its popcount goes through LDS
-    // (one barrier).  The wave then takes its non-zero words (found by a
-    // ballot) two at a time, one bit per lane: set bits get
-    // their rank by ballot, so each step stores a coalesced run of C.  Two
-    // steps are in flight together, and each value is read and cleared by
-    // one LDS exchange.
+    // (one barrier).  A group of sparse words is emitted one set bit per
+    // lane; otherwise the wave takes its non-zero words (found by a ballot)
+    // two at a time, one bit per lane, and set bits get their rank by
+    // ballot.  Either way each step stores a coalesced run of C, and each
+    // value is read and cleared by one LDS exchange.
     const int mwords = (d.cwidth + 31) >> 5;
     const int wpw = (mwords + T / kWave - 1) / (T / kWave);
     const int wv0 = wave_id() * wpw;
@@ -1764,6 +1768,48 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
       };
       int32_t* colp = col_c + d.off;
       V* valp = val_c + d.off;
+      const int32_t wcnt = __popc(word);
+      const int32_t winc = wave_incl_scan(wcnt);
+      const int32_t gtot = wave_last(winc);  // set bits of the group
+      if (gtot < kSparseEmitBits * __popcll(nz)) {
+        // sparse words: lane l emits the group's set bit o0 + l.  Its word
+        // is the last lane j with wex_j <= o (empty words share the next
+        // word's wex, so the non-empty one wins), found by binary lifting
+        // over the lanes; its bit is the k-th set bit of that word, found
+        // by halving popcounts.  Every step stores 64 consecutive entries.
+        const int32_t wex = winc - wcnt;
+        for (int32_t o0 = 0; o0 < gtot; o0 += kWave) {
+          const int32_t o = o0 + lane_id();
+          int j = 0;
+#pragma unroll
+          for (int s = kWave / 2; s >= 1; s >>= 1) {
+            const int32_t v = __shfl(wex, j + s, kWave);
+            j = v <= o ? j + s : j;
+          }
+          uint32_t y = (uint32_t)__shfl((int)word, j, kWave);
+          int32_t k = o - __shfl(wex, j, kWave);
+          int pos = 0;
+#pragma unroll
+          for (int h = 16; h >= 1; h >>= 1) {
+            const int32_t c = __popc(y & ((1u << h) - 1u));
+            const bool up = k >= c;
+            k = up ? k - c : k;
+            y = up ? y >> h : y;
+            pos = up ? pos + h : pos;
+          }
+          const bool valid = o < gtot;
+          const int u = ((g + j) << 5) + pos;
+          const int32_t r = base + o;
+          V v = V(0);
+          if (valid) v = lds_take(&dense[u]);
+          if (valid && r < d.cnt) {
+            colp[r] = d.colbase + u;
+            valp[r] = v;
+          }
+        }
+        base += gtot;
+        continue;
+      }
       while (nz) {
         const int i0 = take(), i1 = take(), i2 = take(), i3 = take();
         const int ia = half ? i1 : i0, ib = half ? i3 : i2;
