@@ -184,7 +184,11 @@ __global__ __launch_bounds__(256) void k_row_flops(Csr A, Csr B, int64_t* row_fl
 }
 
 // One wave per row of A (rows longer than kFlpShort on average); also
-// accumulates P into *total.
+// accumulates P into *total.  A row's nonzeros go in strides of 8*64: the
+// stride's A columns, then its B bounds, are all loaded before any is used,
+// so a hub row of 10^5 nonzeros (which sets the kernel's time) takes a few
+// hundred load latencies instead of a few thousand.
+constexpr int kFlpUnroll = 8;
 __global__ __launch_bounds__(256) void k_row_flops_wave(Csr A, Csr B, int64_t* row_flp,
                                                         unsigned long long* total) {
   __shared__ int64_t s_w[4];
@@ -193,9 +197,22 @@ __global__ __launch_bounds__(256) void k_row_flops_wave(Csr A, Csr B, int64_t* r
   for (int64_t r = blockIdx.x * 4 + wave_id(); r < A.m; r += waves) {
     const int64_t s = A.rp[r], e = A.rp[r + 1];
     int64_t f = 0;
-    for (int64_t p = s + lane_id(); p < e; p += kWave) {
-      const int32_t k = A.col[p];
-      f += B.rp[k + 1] - B.rp[k];
+    for (int64_t p0 = s + lane_id(); p0 < e; p0 += kWave * kFlpUnroll) {
+      int32_t k[kFlpUnroll];
+#pragma unroll
+      for (int j = 0; j < kFlpUnroll; ++j) {
+        const int64_t p = p0 + (int64_t)j * kWave;
+        k[j] = p < e ? A.col[p] : -1;
+      }
+      int64_t lo[kFlpUnroll], hi[kFlpUnroll];
+#pragma unroll
+      for (int j = 0; j < kFlpUnroll; ++j) {
+        const int32_t kk = k[j] < 0 ? 0 : k[j];
+        lo[j] = B.rp[kk];
+        hi[j] = B.rp[kk + 1];
+      }
+#pragma unroll
+      for (int j = 0; j < kFlpUnroll; ++j) f += k[j] < 0 ? 0 : hi[j] - lo[j];
     }
     f = wave_sum(f);
     if (lane_id() == 0) {
