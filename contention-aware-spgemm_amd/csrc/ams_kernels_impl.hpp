@@ -704,12 +704,14 @@ __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, Expan
 #ifndef AMS_SYMB_WPE
 #define AMS_SYMB_WPE 8
 #endif
+constexpr int kSymGrab = 4;
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE))) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
                                                    const int32_t* list, const int32_t* counts, int bin,
                                                    int64_t* rownz, int32_t* coff, int R, int W, int Wb,
                                                    const uint8_t* hflag, const HeavyPiece* pieces,
-                                                   const int32_t* npieces, uint32_t* gbm, int64_t gwords) {
+                                                   const int32_t* npieces, uint32_t* gbm, int64_t gwords,
+                                                   unsigned long long* row_ticket) {
   extern __shared__ uint32_t dyn[];
   uint32_t* bm = dyn;
   int32_t* ccount = (int32_t*)(dyn + (Wb >> 5));
@@ -744,7 +746,26 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       __syncthreads();
     }
   }
-  for (int32_t h = blockIdx.x; h < cnt; h += gridDim.x) {
+  // then the rows, up to kSymGrab tickets at a time from the global counter
+  // (the bin list's order, and so the rows' sizes, vary from run to run); one
+  // at a time when the bin has few rows per block
+  __shared__ int32_t s_next;
+  const unsigned long long per_block = (unsigned long long)cnt / ((unsigned long long)gridDim.x * 16ull);
+  const unsigned long long grab = per_block < 1 ? 1 : (per_block > kSymGrab ? kSymGrab : per_block);
+  unsigned long long q_tk = 0, q_end = 0;  // thread 0's batch of tickets
+  for (;;) {
+    if (threadIdx.x == 0) {
+      if (q_tk == q_end) {
+        q_tk = atomicAdd(row_ticket, grab);
+        q_end = q_tk + grab;
+      }
+      s_next = q_tk < (unsigned long long)cnt ? (int32_t)q_tk : cnt;
+      ++q_tk;
+    }
+    __syncthreads();
+    const int32_t h = s_next;
+    __syncthreads();  // every thread has read s_next before thread 0 writes it again
+    if (h >= cnt) break;
     const int32_t row = list[start + h];
     int64_t lo, width;
     prob_cols(pr, row, lo, width);

@@ -47,7 +47,8 @@ struct Ctrl {
   unsigned long long sums[16];
   int32_t nheavy;   // heavy rows of the bitmap bin split into pieces
   int32_t npieces;
-  int64_t pad2[4];
+  unsigned long long sym_ticket;  // next row of the bitmap bin to count
+  int64_t pad2[3];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 
@@ -412,10 +413,12 @@ static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const 
   int occ;
   occupancy(k_symb_bitmap<T>, T, dyn, &occ);
   const int g = grid_for(ctx, occ, H, ctx->plan_counts);
+  // c->sym_ticket starts at zero: the control block is cleared at the start
+  // of every symbolic phase, which launches this kernel once
   hipLaunchKernelGGL(k_symb_bitmap<T>, dim3(g), dim3(T), dyn, ctx->stream, A, B, pr, (const int64_t*)ctx->row_flp.p,
                      (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
                      (int32_t*)ctx->coff.p, R, W, Wb, hflag, (const HeavyPiece*)ctx->hpiece.p,
-                     (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p, gwords);
+                     (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p, gwords, &c->sym_ticket);
   return AMS_OK;
 }
 
