@@ -673,11 +673,21 @@ __global__ __launch_bounds__(T) void k_heavy_finish(const Probs pr, const int32_
 // nonzeros [p0, p1) in the LDS bitmap bm.  Hub columns recur: the bit is
 // tested first, so that only unset bits take the (same-word serialised)
 // atomic; a stride's bit reads all go out before the first test.
-template <int T>
+// Q32: B has fewer than 2^30 nonzeros, so a column's byte offset fits 32
+// bits and the load takes the scalar base plus a 32-bit vector offset.
+template <int T, bool Q32>
 __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, const SegB& seg,
                                                    const Csr& B, uint32_t* bm, int64_t base, int32_t pw) {
   block_expand_batch<T, kUSymBitmap>(
-      p0, p1, L, seg, [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+      p0, p1, L, seg,
+      [&](int, int64_t, int64_t q, int64_t) {
+        if constexpr (Q32) {
+          const uint32_t boff = (uint32_t)q << 2;
+          return *(const int32_t*)((const char*)B.col + boff);
+        } else {
+          return B.col[q];
+        }
+      },
       [&](int32_t (&col)[kUSymBitmap], bool (&ok)[kUSymBitmap]) {
         uint32_t u[kUSymBitmap], w[kUSymBitmap];
 #pragma unroll
@@ -705,7 +715,7 @@ __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, Expan
 #define AMS_SYMB_WPE 8
 #endif
 constexpr int kSymGrab = 4;
-template <int T>
+template <int T, bool Q32>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE))) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
                                                    const int32_t* list, const int32_t* counts, int bin,
                                                    int64_t* rownz, int32_t* coff, int R, int W, int Wb,
@@ -738,7 +748,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       for (int w = threadIdx.x; w < words; w += T) bm[w] = 0u;
       __syncthreads();
       const int64_t base = lo + c0;
-      symb_bitmap_expand<T>(q0, q1, L, seg, B, bm, base, pw);
+      symb_bitmap_expand<T, Q32>(q0, q1, L, seg, B, bm, base, pw);
       for (int w = threadIdx.x; w < words; w += T) {
         const uint32_t x = bm[w];
         if (x) atomicOr(&g[(c0 >> 5) + w], x);
@@ -781,7 +791,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
-      symb_bitmap_expand<T>(A.rp[row], A.rp[row + 1], L, seg, B, bm, base, pw);
+      symb_bitmap_expand<T, Q32>(A.rp[row], A.rp[row + 1], L, seg, B, bm, base, pw);
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {
         const int w = w0 + threadIdx.x;
