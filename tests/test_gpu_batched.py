@@ -90,3 +90,29 @@ def test_batched_autotune_keeps_results(gpu_ctx):
         assert_matches_oracle(c, a, b, exact=False)
     bp.close()
     ctx.close()
+
+
+def test_batched_heavy_rows(gpu_ctx):
+    """A batch holding a product with rows of more than max(2^20, P/2048)
+    products (a join-shaped C = A^T A over 300 columns) between small
+    ones: the heavy rows are counted in pieces by the batch's narrow bitmap
+    blocks, each problem at its own column offset, on every replay."""
+    import torch
+    from ams_hip.batched import BatchedProduct
+    from ams_hip.device import Context
+    inc = gen.join_incidence(num_row=60000, num_col=300, mu=3.5, sigma=0.5, cap=100, seed=5)
+    probs = small_batch(6, seed=31)
+    probs.insert(3, (inc.transpose(), inc))
+    stream = torch.cuda.Stream()
+    ctx = Context(stream=stream)
+    bp = BatchedProduct(probs, ams=ctx)
+    for rep in range(2):
+        with torch.cuda.stream(stream):
+            bp.c_val.fill_(float("nan"))
+            bp.run()
+        res = bp.results()
+        for (a, b), c in zip(probs, res):
+            assert c.sorted_rows()
+            assert_matches_oracle(c, a, b, exact=False)
+    bp.close()
+    ctx.close()
