@@ -34,7 +34,11 @@ enum { AMS_F64 = 0, AMS_F32 = 1 };
 enum { AMS_MODE_HYBRID = 0, AMS_MODE_HASH = 1, AMS_MODE_ESC = 2 };
 
 /* A CSR matrix in device memory (csr_t of src/inc/store.h:49-56, with
- * device index widths). */
+ * device index widths).  Like the reference, the library trusts the
+ * entries: row_ptr[0] = 0, row_ptr non-decreasing, row_ptr[num_row] =
+ * num_nnz and every column in [0, num_col) are the caller's to ensure (the
+ * calls check sizes and pointers only).  Columns of a row may come in any
+ * order and may repeat; each copy adds its products. */
 typedef struct {
 	int64_t num_row;
 	int64_t num_col;
