@@ -269,8 +269,13 @@ struct Phase {
 };
 
 // Geometry of the dense column chunks and the symbolic bitmap passes.
+// dense_width is the FP64 chunk width; FP32 chunks are 7/4 as wide (the
+// 4-byte accumulators leave the LDS room, and fewer, larger morsels pay
+// less per-morsel overhead: 95.9 -> 85.0 ms of FP32 dense numeric on G500
+// s18 ef16; twice as wide no longer fits two blocks per CU).
 static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W, int& R, int& Wb) {
   W = ctx->prm.dense_width;
+  if (dtype == AMS_F32) W = (int)((int64_t)W * 7 / 4 / 64 * 64);
   if (max_width > 0 && W > max_width) W = (int)((max_width + 63) / 64 * 64);
   W = std::max(64, std::min(W, 65536));
   R = (int)((max_width + W - 1) / W);
@@ -279,7 +284,6 @@ static void dense_geometry(ams_ctx_t* ctx, int dtype, int64_t max_width, int& W,
   int64_t wb = std::min<int64_t>((int64_t)R * W, (int64_t)ctx->prm.bitmap_bits / W * W);
   if (wb < W) wb = W;
   Wb = (int)wb;
-  (void)dtype;
 }
 
 // Words of a heavy row's global bitmap (the widest problem's columns), or 0

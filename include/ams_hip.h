@@ -55,7 +55,8 @@ typedef struct {
 	int32_t num_hash_max;   /* rows with nnz(C row) <= this may use an LDS accumulator */
 	int32_t esc_max;        /* rows with flops <= this may use ESC                   */
 	double  acc_thres;      /* flops / nnz(C row) below this -> ESC (AMS_ACC_THRES)  */
-	int32_t dense_width;    /* columns per dense LDS chunk (multiple of 64)           */
+	int32_t dense_width;    /* columns per dense LDS chunk (multiple of 64) for FP64;
+	                           FP32 chunks are 7/4 as wide, rounded down to 64     */
 	int32_t bitmap_bits;    /* LDS bitmap width of the heavy symbolic, bits           */
 	double  bitmap_ratio;   /* bitmap symbolic when flops >= ratio * words(n)         */
 	double  dense_density;  /* dense bin also when nnz(C row) >= density * n          */
