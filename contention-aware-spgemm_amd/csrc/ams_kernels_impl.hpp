@@ -1178,14 +1178,26 @@ __device__ __forceinline__ int32_t wave_next(int32_t x, int32_t last) {
 }
 
 // True when the columns of B row [s, e) never decrease.
+// Hub rows of B have about 10^5 entries: both walks below load kRowGroups
+// 64-entry groups before they use the first, so that a long row takes a
+// quarter of the load latencies in a row.
+constexpr int kRowGroups = 4;
+
 __device__ __forceinline__ bool row_ascending(const int32_t* col, int64_t s, int64_t e) {
   bool bad = false;
   int32_t before = INT32_MIN;  // last column of the previous group
-  for (int64_t q0 = s; q0 < e; q0 += kWave) {
-    const int64_t q = q0 + lane_id();
-    const int32_t x = q < e ? col[q] : INT32_MAX;
-    bad |= wave_prev(x, before) > x;
-    before = __builtin_amdgcn_readlane(x, kWave - 1);
+  for (int64_t q0 = s; q0 < e; q0 += kWave * kRowGroups) {
+    int32_t x[kRowGroups];
+#pragma unroll
+    for (int g = 0; g < kRowGroups; ++g) {
+      const int64_t q = q0 + (int64_t)g * kWave + lane_id();
+      x[g] = q < e ? col[q] : INT32_MAX;
+    }
+#pragma unroll
+    for (int g = 0; g < kRowGroups; ++g) {
+      bad |= wave_prev(x[g], before) > x[g];
+      before = __builtin_amdgcn_readlane(x[g], kWave - 1);
+    }
   }
   return !__any(bad);
 }
@@ -1209,21 +1221,31 @@ template <class F>
 __device__ __forceinline__ void row_runs(const int32_t* col, int64_t s, int64_t e, int32_t lo, int W, F f) {
   int32_t carry = 0;   // 1 + row-local index of the latest run start
   int32_t cprev = -1;  // chunk of the previous group's last entry
-  for (int64_t q0 = s; q0 < e; q0 += kWave) {
-    const int64_t q = q0 + lane_id();
-    const bool valid = q < e;
-    const int32_t loc = valid ? col[q] - lo : 0;
-    const int32_t c = valid ? loc / W : -1;
-    int32_t cn = -1;
-    if (lane_id() == kWave - 1 && q + 1 < e) cn = (col[q + 1] - lo) / W;
-    const int32_t cp = wave_prev(c, cprev);
-    cn = wave_next(c, cn);
-    const bool start = valid && cp != c;
-    int32_t m = wave_incl_max_nonneg(start ? (int32_t)(q - s + 1) : 0);
-    m = m > carry ? m : carry;
-    carry = __builtin_amdgcn_readlane(m, kWave - 1);
-    cprev = __builtin_amdgcn_readlane(c, kWave - 1);
-    if (valid) f(q, c, loc, (int64_t)(m - 1), q + 1 == e || cn != c);
+  for (int64_t g0 = s; g0 < e; g0 += kWave * kRowGroups) {
+    int32_t x[kRowGroups], xn[kRowGroups];  // columns, and lane 63's next one
+#pragma unroll
+    for (int g = 0; g < kRowGroups; ++g) {
+      const int64_t q = g0 + (int64_t)g * kWave + lane_id();
+      x[g] = q < e ? col[q] : 0;
+      xn[g] = (lane_id() == kWave - 1 && q + 1 < e) ? col[q + 1] : 0;
+    }
+#pragma unroll
+    for (int g = 0; g < kRowGroups; ++g) {
+      const int64_t q = g0 + (int64_t)g * kWave + lane_id();
+      const bool valid = q < e;
+      const int32_t loc = valid ? x[g] - lo : 0;
+      const int32_t c = valid ? loc / W : -1;
+      int32_t cn = -1;
+      if (lane_id() == kWave - 1 && q + 1 < e) cn = (xn[g] - lo) / W;
+      const int32_t cp = wave_prev(c, cprev);
+      cn = wave_next(c, cn);
+      const bool start = valid && cp != c;
+      int32_t m = wave_incl_max_nonneg(start ? (int32_t)(q - s + 1) : 0);
+      m = m > carry ? m : carry;
+      carry = __builtin_amdgcn_readlane(m, kWave - 1);
+      cprev = __builtin_amdgcn_readlane(c, kWave - 1);
+      if (valid) f(q, c, loc, (int64_t)(m - 1), q + 1 == e || cn != c);
+    }
   }
 }
 
