@@ -715,7 +715,12 @@ __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, Expan
 #define AMS_SYMB_WPE 8
 #endif
 constexpr int kSymGrab = 4;
-template <int T, bool Q32>
+// SW > 0: the bitmap and chunk counters live in a static LDS array of SW
+// words (its address is known when compiling, so the word offsets need no
+// base register), used by 1024-thread blocks, which two waves-limited
+// blocks per CU leave 80 KB of LDS each.
+constexpr int kSymStaticWords = 17152;
+template <int T, bool Q32, int SW>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE))) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
                                                    const int32_t* list, const int32_t* counts, int bin,
                                                    int64_t* rownz, int32_t* coff, int R, int W, int Wb,
@@ -723,8 +728,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
                                                    const int32_t* npieces, uint32_t* gbm, int64_t gwords,
                                                    unsigned long long* row_ticket) {
   extern __shared__ uint32_t dyn[];
-  uint32_t* bm = dyn;
-  int32_t* ccount = (int32_t*)(dyn + (Wb >> 5));
+  __shared__ uint32_t s_bm[SW > 0 ? SW : 1];
+  uint32_t* bm = SW > 0 ? s_bm : dyn;
+  int32_t* ccount = (int32_t*)(bm + (Wb >> 5));
   __shared__ ExpandLds<T> L;
   int32_t start, cnt;
   bin_range(counts, bin, start, cnt);

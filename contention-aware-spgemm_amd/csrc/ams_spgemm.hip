@@ -407,20 +407,21 @@ static void launch_symb_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t
 
 // The bitmap bin's wide rows (and heavy-row pieces, when hflag is set), one
 // row per T-thread block with dyn bytes of bitmap and chunk counters.
-template <int T, bool Q32>
+template <int T, bool Q32, int SW>
 static int launch_symb_bitmap_q(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
                                 int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords) {
-  if (dyn + sizeof(ExpandLds<T>) > ctx->lds_max)
+  if (SW > 0) dyn = 0;  // the bitmap is the kernel's static array
+  if (dyn + sizeof(ExpandLds<T>) + (size_t)SW * 4 > ctx->lds_max)
     return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
-  HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<T, Q32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<T, Q32, SW>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)dyn));
   Ctrl* c = dctrl(ctx);
   int occ;
-  occupancy(k_symb_bitmap<T, Q32>, T, dyn, &occ);
+  occupancy(k_symb_bitmap<T, Q32, SW>, T, dyn, &occ);
   const int g = grid_for(ctx, occ, H, ctx->plan_counts);
   // c->sym_ticket starts at zero: the control block is cleared at the start
   // of every symbolic phase, which launches this kernel once
-  hipLaunchKernelGGL((k_symb_bitmap<T, Q32>), dim3(g), dim3(T), dyn, ctx->stream, A, B, pr,
+  hipLaunchKernelGGL((k_symb_bitmap<T, Q32, SW>), dim3(g), dim3(T), dyn, ctx->stream, A, B, pr,
                      (const int64_t*)ctx->row_flp.p,
                      (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
                      (int32_t*)ctx->coff.p, R, W, Wb, hflag, (const HeavyPiece*)ctx->hpiece.p,
@@ -431,9 +432,15 @@ static int launch_symb_bitmap_q(ams_ctx_t* ctx, const Csr& A, const Csr& B, cons
 template <int T>
 static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
                               int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords) {
-  if (B.nnz < ((int64_t)1 << 30))
-    return launch_symb_bitmap_q<T, true>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords);
-  return launch_symb_bitmap_q<T, false>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords);
+  const bool q32 = B.nnz < ((int64_t)1 << 30);
+  if constexpr (T == 1024) {
+    if ((int64_t)(Wb >> 5) + Wb / W + 1 <= kSymStaticWords)
+      return q32 ? launch_symb_bitmap_q<T, true, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords)
+                 : launch_symb_bitmap_q<T, false, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag,
+                                                                   gwords);
+  }
+  return q32 ? launch_symb_bitmap_q<T, true, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords)
+             : launch_symb_bitmap_q<T, false, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords);
 }
 
 static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype,
