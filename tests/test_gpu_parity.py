@@ -253,3 +253,21 @@ def test_config1_full(gpu_ctx):
     st = gpu_ctx.stats()
     print(st, file=sys.stderr)
     assert st["nnz_c"] == c.num_nnz
+
+
+def test_fp32_chunk_width(gpu_ctx):
+    """FP32 dense chunks are 7/4 as wide as FP64 ones (rounded down to 64
+    columns): the same wide product runs in fewer FP32 chunks, both exact."""
+    a64 = gen.rmat_symmetric(15, 4, seed=61)
+    a32 = a64.astype(np.int32, np.float32)
+    params = dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_density=0.0, dense_width=8448)
+    c64 = run(gpu_ctx, a64, a64, params)
+    r64 = gpu_ctx.stats()["dense_chunks"]
+    c32 = run(gpu_ctx, a32, a32, params)
+    r32 = gpu_ctx.stats()["dense_chunks"]
+    n = a64.num_col
+    assert r64 == -(-n // 8448) and r32 == -(-n // (8448 * 7 // 4 // 64 * 64)), (r64, r32)
+    assert_matches_oracle(c64, a64, a64, exact=True)
+    np.testing.assert_array_equal(c32.row_ptrs, c64.row_ptrs)
+    np.testing.assert_array_equal(c32.cols, c64.cols)
+    np.testing.assert_array_equal(c32.rats, c64.rats.astype(np.float32))
