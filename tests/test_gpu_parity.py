@@ -134,6 +134,26 @@ def test_wide_heavy_rows(gpu_ctx):
     assert_matches_oracle(c, a, b, exact=True)
 
 
+def test_wide_bitmap_dynamic_lds(gpu_ctx):
+    """Bitmap rows over 700000 columns: a bitmap pass of 83 chunks (21912
+    words) is too large for the symbolic kernel's static LDS array, so the
+    dynamic-LDS instantiation counts them."""
+    rng = np.random.default_rng(71)
+    n, kb, nb = 700000, 300, 2000
+    cols = np.sort(rng.integers(0, n, size=(nb, kb)), axis=1)
+    keep = np.ones_like(cols, dtype=bool)
+    keep[:, 1:] = cols[:, 1:] != cols[:, :-1]
+    rp = np.concatenate([[0], np.cumsum(keep.sum(axis=1))]).astype(np.int64)
+    b = Csr(nb, n, rp, cols[keep].astype(np.int32), np.ones(int(rp[-1])))
+    arows = [np.sort(rng.choice(nb, size=k, replace=False)) for k in (200, 150, 5, 90)]
+    a_rp = np.concatenate([[0], np.cumsum([len(r) for r in arows])]).astype(np.int64)
+    a = Csr(len(arows), nb, a_rp, np.concatenate(arows).astype(np.int32), np.ones(int(a_rp[-1])))
+    print("[parity] bitmap rows over 700000 columns", flush=True)
+    c = run(gpu_ctx, a, b, dict(sym_hash_max=1, bitmap_ratio=0.0))
+    assert c.sorted_rows()
+    assert_matches_oracle(c, a, b, exact=True)
+
+
 @pytest.mark.parametrize("route", ["default", "esc_only"])
 @pytest.mark.parametrize("b_density", [0.0005, 0.002, 0.0055])
 def test_esc_long_a_rows(gpu_ctx, route, b_density):
