@@ -16,14 +16,12 @@ import torch
 
 from .device import Context, DeviceCsr
 
-_default_ctx = None
-
 
 def default_context() -> Context:
-    global _default_ctx
-    if _default_ctx is None:
-        _default_ctx = Context()
-    return _default_ctx
+    """The library context of the current stream (one per stream), so the
+    product runs in order with the caller's work on that stream."""
+    from .sparse import context
+    return context()
 
 
 class _SpGEMM(torch.autograd.Function):

@@ -109,9 +109,17 @@ class BatchedProduct:
         else:
             self.stream.synchronize()
 
+    def check(self):
+        """Raises when the kernels of the last run raised error flags."""
+        flags = self.ams.error_flags()
+        if flags:
+            raise L.AmsError(f"batched run raised kernel error flags 0x{flags:x}")
+
     def results(self) -> List[Csr]:
-        """Per-problem products (host copies, columns rebased)."""
+        """Per-problem products (host copies, columns rebased) of the last
+        run, after checking its kernel error flags."""
         self.synchronize()
+        self.check()
         rp = self.c_rp.cpu().numpy()
         nnz = int(rp[-1])
         cols = self.c_col[:nnz].cpu().numpy().astype(np.int64)

@@ -164,6 +164,13 @@ class Context:
                                               C.byref(sval), C.byref(uns)), "ams_verify_result")
         return {"nnz": nnz.value, "sum_col": scol.value, "sum_val": sval.value, "unsorted_rows": uns.value}
 
+    def error_flags(self) -> int:
+        """Error flags raised by the kernels of the last run on this
+        context (0 = none), read after its stream drains."""
+        f = C.c_int32()
+        self._chk(self._lib.ams_get_error_flags(self._p, C.byref(f)), "ams_get_error_flags")
+        return f.value
+
     def autotune(self, a: DeviceCsr, b: DeviceCsr, reps: int = 2):
         best, ms = L.AmsParams(), C.c_double()
         sa, sb = a.c_struct(), b.c_struct()

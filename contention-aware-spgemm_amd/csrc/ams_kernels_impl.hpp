@@ -521,7 +521,10 @@ __global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* li
 
 // Bitmap rows of narrow problems with few products do not fill a block: a
 // wave takes each of them, with a private LDS bitmap of the row's width and
-// no block barriers.  k_symb_bitmap skips these rows.
+// no block barriers.  k_symb_bitmap skips these rows when told (wave_rows)
+// that this kernel runs; the host launches it whenever a problem of at most
+// kWaveBitmapBits columns may exist (every batch, and single problems that
+// narrow), so the two kernels always split the bin between them.
 constexpr int kWaveBitmapBits = 16384;
 constexpr int64_t kWaveBitmapFlops = 8192;
 
@@ -726,7 +729,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
                                                    int64_t* rownz, int32_t* coff, int R, int W, int Wb,
                                                    const uint8_t* hflag, const HeavyPiece* pieces,
                                                    const int32_t* npieces, uint32_t* gbm, int64_t gwords,
-                                                   unsigned long long* row_ticket) {
+                                                   unsigned long long* row_ticket, int wave_rows) {
   extern __shared__ uint32_t dyn[];
   __shared__ uint32_t s_bm[SW > 0 ? SW : 1];
   uint32_t* bm = SW > 0 ? s_bm : dyn;
@@ -785,7 +788,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
     const int32_t row = list[start + h];
     int64_t lo, width;
     prob_cols(pr, row, lo, width);
-    if (width <= kWaveBitmapBits && row_flp[row] <= kWaveBitmapFlops) continue;  // k_symb_bitmap_wave's rows
+    // k_symb_bitmap_wave's rows, when the host launches it (wave_rows)
+    if (wave_rows && width <= kWaveBitmapBits && row_flp[row] <= kWaveBitmapFlops) continue;
     if (hflag && hflag[h]) continue;  // split into pieces above
     int32_t running = 0;
     int32_t* crow = coff + (int64_t)h * (R + 1);

@@ -114,11 +114,28 @@ struct ams_ctx {
   std::vector<Rec> recs;
   double acc_ms[AMS_PH_COUNT] = {};
   int64_t acc_n[AMS_PH_COUNT] = {};
+  // operands of the plan: the numeric phase must see the same matrices
+  int64_t plan_nnzA = 0, plan_nB = 0;
+  const void *plan_arp = nullptr, *plan_acol = nullptr, *plan_brp = nullptr, *plan_bcol = nullptr;
   // batched plan
   bool batch_ready = false;
   Probs batch_pr{};
   int64_t batch_m = 0, batch_mB = 0, batch_nnzA = 0, batch_nnzB = 0;
   int batch_dtype = 0;
+  // Generation of the scratch layout: bumped whenever a buffer moves
+  // (reallocation) or a batched plan rewrites its problem tables.  A
+  // captured graph bakes in buffer addresses and replays only while the
+  // generation it was captured at is current.
+  uint64_t gen = 1;
+  bool capturing = false;  // inside ams_batched_capture: buffers must not move
+};
+
+// Handle of a captured batched run: the executable graph, the context it
+// belongs to and the scratch generation its kernel arguments point into.
+struct AmsGraph {
+  hipGraphExec_t exec;
+  const ams_ctx_t* ctx;
+  uint64_t gen;
 };
 
 static int fail(ams_ctx_t* ctx, int code, const char* fmt, ...) {
@@ -148,7 +165,11 @@ static int fail(ams_ctx_t* ctx, int code, const char* fmt, ...) {
 static int grow(ams_ctx_t* ctx, Buf& b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (b.cap >= bytes) return AMS_OK;
+  if (ctx->capturing)
+    return fail(ctx, AMS_ERR_STATE, "a scratch buffer of %zu bytes would move during graph capture; "
+                "call ams_batched_plan for these inputs first", bytes);
   if (b.p) {
+    ++ctx->gen;  // the buffer moves: graphs captured before are stale
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipFree(b.p));
     b.p = nullptr;
@@ -409,7 +430,8 @@ static void launch_symb_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t
 // row per T-thread block with dyn bytes of bitmap and chunk counters.
 template <int T, bool Q32, int SW>
 static int launch_symb_bitmap_q(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
-                                int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords) {
+                                int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords,
+                                int wave_rows) {
   if (SW > 0) dyn = 0;  // the bitmap is the kernel's static array
   if (dyn + sizeof(ExpandLds<T>) + (size_t)SW * 4 > ctx->lds_max)
     return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
@@ -425,22 +447,24 @@ static int launch_symb_bitmap_q(ams_ctx_t* ctx, const Csr& A, const Csr& B, cons
                      (const int64_t*)ctx->row_flp.p,
                      (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
                      (int32_t*)ctx->coff.p, R, W, Wb, hflag, (const HeavyPiece*)ctx->hpiece.p,
-                     (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p, gwords, &c->sym_ticket);
+                     (const int32_t*)&c->npieces, (uint32_t*)ctx->gbm.p, gwords, &c->sym_ticket, wave_rows);
   return AMS_OK;
 }
 
 template <int T>
 static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
-                              int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords) {
+                              int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords,
+                              int wave_rows) {
   const bool q32 = B.nnz < ((int64_t)1 << 30);
   if constexpr (T == 1024) {
     if ((int64_t)(Wb >> 5) + Wb / W + 1 <= kSymStaticWords)
-      return q32 ? launch_symb_bitmap_q<T, true, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords)
+      return q32 ? launch_symb_bitmap_q<T, true, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords,
+                                                                  wave_rows)
                  : launch_symb_bitmap_q<T, false, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag,
-                                                                   gwords);
+                                                                   gwords, wave_rows);
   }
-  return q32 ? launch_symb_bitmap_q<T, true, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords)
-             : launch_symb_bitmap_q<T, false, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords);
+  return q32 ? launch_symb_bitmap_q<T, true, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords, wave_rows)
+             : launch_symb_bitmap_q<T, false, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords, wave_rows);
 }
 
 static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype,
@@ -467,6 +491,12 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   ctx->plan_nnzB = B.nnz;
   ctx->plan_dtype = dtype;
   ctx->plan_pr = pr;
+  ctx->plan_nnzA = A.nnz;
+  ctx->plan_nB = B.n;
+  ctx->plan_arp = A.rp;
+  ctx->plan_acol = A.col;
+  ctx->plan_brp = B.rp;
+  ctx->plan_bcol = B.col;
 
   SymParams sp{ctx->prm.sym_hash_max, ctx->prm.bitmap_ratio, Wb};
   if (m > 0) {
@@ -555,10 +585,13 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
         HIPCHK(hipGetLastError());
         hflag = (const uint8_t*)ctx->hflag.p;
       }
+      // light rows of problems of at most kWaveBitmapBits columns go to the
+      // wave kernel below; in a batch any problem may be that narrow
+      const int wave_rows = (pr.n > 1 || pr.max_width <= kWaveBitmapBits) ? 1 : 0;
       if (pr.n > 1 && Wb <= kSymNarrowBits)
-        CHK(launch_symb_bitmap<kSymBitmapNarrowT>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords));
+        CHK(launch_symb_bitmap<kSymBitmapNarrowT>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords, wave_rows));
       else
-        CHK(launch_symb_bitmap<kSymBitmapT>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords));
+        CHK(launch_symb_bitmap<kSymBitmapT>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords, wave_rows));
       if (split) {
         // k_heavy_finish clears what the pieces set; should either launch
         // fail, the global bitmaps are refilled with zeros on the next call
@@ -574,7 +607,7 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
         ctx->gbm_zero = ctx->gbm.cap;
       }
       HIPCHK(hipGetLastError());
-      if (pr.max_width <= kWaveBitmapBits) {  // light rows of narrow problems: a wave each
+      if (wave_rows) {  // light rows of narrow problems: a wave each
         constexpr int TW = 256;
         int occw;
         occupancy(k_symb_bitmap_wave<TW>, TW, 0, &occw);
@@ -905,7 +938,7 @@ int ams_ctx_create(ams_ctx_t** out, void* stream) {
   if (hipHostMalloc((void**)&ctx->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_pin, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
-    delete ctx;
+    ams_ctx_destroy(ctx);
     return AMS_ERR_ALLOC;
   }
   *out = ctx;
@@ -1002,14 +1035,43 @@ int ams_spgemm_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   return run_symbolic(ctx, a, b, dtype, single_probs(b), rp_c, true, nnz);
 }
 
+// The routing and column-chunk plan of the last symbolic call describes
+// exactly these operands: same sizes, same row-pointer and column arrays
+// (values may differ).
+static bool plan_matches(const ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype) {
+  return ctx->plan_valid && ctx->plan_m == a->num_row && ctx->plan_mB == b->num_row && ctx->plan_dtype == dtype &&
+         ctx->plan_nnzA == a->num_nnz && ctx->plan_nnzB == b->num_nnz && ctx->plan_nB == b->num_col &&
+         ctx->plan_arp == a->row_ptr && ctx->plan_acol == a->col && ctx->plan_brp == b->row_ptr &&
+         ctx->plan_bcol == b->col;
+}
+
+// Reads the kernels' error flag word after the stream drains.
+static int check_flags(ams_ctx_t* ctx, const char* what) {
+  HIPCHK(hipMemcpyAsync(&ctx->h_ctrl->err, &dctrl(ctx)->err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "%s kernels raised error flags 0x%x", what, ctx->h_ctrl->err);
+  return AMS_OK;
+}
+
 int ams_spgemm_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, const int64_t* rp_c,
                        int32_t* col_c, void* val_c) {
   CHK(check_pair(ctx, a, b, dtype));
-  if (!ctx->plan_valid || ctx->plan_m != a->num_row || ctx->plan_mB != b->num_row || ctx->plan_dtype != dtype)
-    return fail(ctx, AMS_ERR_STATE, "numeric needs the symbolic phase of the same A and B first");
+  if (!plan_matches(ctx, a, b, dtype))
+    return fail(ctx, AMS_ERR_STATE, "numeric needs the symbolic phase of the same A and B (same arrays) first");
   if (!rp_c || (ctx->stats.nnz_c > 0 && (!col_c || !val_c)))
     return fail(ctx, AMS_ERR_ARG, "output buffers are NULL");
-  return run_numeric(ctx, a, b, dtype, rp_c, col_c, val_c, INT64_MAX);
+  CHK(run_numeric(ctx, a, b, dtype, rp_c, col_c, val_c, INT64_MAX));
+  return check_flags(ctx, "numeric");
+}
+
+int ams_get_error_flags(ams_ctx_t* ctx, int32_t* flags) {
+  if (!ctx || !flags) return AMS_ERR_ARG;
+  *flags = 0;
+  if (!ctx->ctrl.p) return AMS_OK;
+  HIPCHK(hipMemcpyAsync(&ctx->h_ctrl->err, &dctrl(ctx)->err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *flags = ctx->h_ctrl->err;
+  return AMS_OK;
 }
 
 int ams_spgemm(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, ams_dcsr_t* c_out) {
@@ -1023,9 +1085,7 @@ int ams_spgemm(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dty
   CHK(grow(ctx, ctx->pool_col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
   CHK(grow(ctx, ctx->pool_val, vb * std::max<int64_t>(nnz, 1)));
   CHK(run_numeric(ctx, a, b, dtype, rp, (int32_t*)ctx->pool_col.p, ctx->pool_val.p, INT64_MAX));
-  HIPCHK(hipMemcpyAsync(&ctx->h_ctrl->err, &dctrl(ctx)->err, 4, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "numeric kernels raised error flags 0x%x", ctx->h_ctrl->err);
+  CHK(check_flags(ctx, "numeric"));
   c_out->num_row = a->num_row;
   c_out->num_col = b->num_col;
   c_out->num_nnz = nnz;
@@ -1211,9 +1271,11 @@ int ams_batched_plan(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, i
   CHK(grow(ctx, ctx->ccnt, sizeof(int32_t) * std::max<int64_t>(m, 1) * R));
   CHK(grow(ctx, ctx->dflag, sizeof(int64_t) * (std::max<int64_t>(m, 1) * R + 1)));
   CHK(grow(ctx, ctx->dtask, sizeof(DenseTask) * std::max<int64_t>(m, 1) * R));
+  CHK(grow(ctx, ctx->lrows, sizeof(int32_t) * std::max<int64_t>(b->num_row, 1)));
   const int64_t nmax = std::max<int64_t>(std::max<int64_t>(m + 1, (int64_t)R * ld), std::max<int64_t>(m, 1) * R + 1);
   CHK(grow(ctx, ctx->tsum, sizeof(int64_t) * ((nmax + kScanTile - 1) / kScanTile)));
   if (heavy_words(pr) > 0) CHK(grow_heavy(ctx, pr, m));
+  ++ctx->gen;  // the problem tables were rewritten: graphs of earlier plans are stale
   ctx->batch_pr = pr;
   ctx->batch_m = m;
   ctx->batch_mB = b->num_row;
@@ -1243,10 +1305,20 @@ int ams_batched_capture(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
                         int32_t* c_col, void* c_val, int64_t cap, int64_t* nnz_dev, void** exec) {
   if (!ctx || !exec) return AMS_ERR_ARG;
   if (!ctx->stream) return fail(ctx, AMS_ERR_ARG, "graph capture needs a non-null stream");
-  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  const bool timing = ctx->timing;
+  ctx->timing = false;  // no phase events inside a graph
+  ctx->capturing = true;
+  hipError_t e0 = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+  if (e0 != hipSuccess) {
+    ctx->capturing = false;
+    ctx->timing = timing;
+    return fail(ctx, AMS_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e0));
+  }
   const int r = ams_batched_run(ctx, a, b, dtype, c_rp, c_col, c_val, cap, nnz_dev);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->capturing = false;
+  ctx->timing = timing;
   if (r != AMS_OK) {
     if (g) (void)hipGraphDestroy(g);
     return r;
@@ -1256,18 +1328,28 @@ int ams_batched_capture(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   const hipError_t e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e2 != hipSuccess) return fail(ctx, AMS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e2));
-  *exec = (void*)ge;
+  AmsGraph* h = new AmsGraph{ge, ctx, ctx->gen};
+  *exec = (void*)h;
   return AMS_OK;
 }
 
 int ams_graph_launch(ams_ctx_t* ctx, void* exec) {
   if (!ctx || !exec) return AMS_ERR_ARG;
-  HIPCHK(hipGraphLaunch((hipGraphExec_t)exec, ctx->stream));
+  const AmsGraph* h = (const AmsGraph*)exec;
+  if (h->ctx != ctx) return fail(ctx, AMS_ERR_ARG, "the graph was captured on another context");
+  if (h->gen != ctx->gen)
+    return fail(ctx, AMS_ERR_STATE, "the graph is stale: the context's scratch moved or was re-planned since its "
+                "capture; capture it again");
+  HIPCHK(hipGraphLaunch(h->exec, ctx->stream));
   return AMS_OK;
 }
 
 int ams_graph_destroy(void* exec) {
-  if (exec && hipGraphExecDestroy((hipGraphExec_t)exec) != hipSuccess) {
+  if (!exec) return AMS_OK;
+  AmsGraph* h = (AmsGraph*)exec;
+  const hipError_t e = hipGraphExecDestroy(h->exec);
+  delete h;
+  if (e != hipSuccess) {
     (void)hipGetLastError();
     return AMS_ERR_HIP;
   }

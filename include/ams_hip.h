@@ -132,8 +132,12 @@ int ams_spgemm_symbolic(ams_ctx_t *ctx, const ams_dcsr_t *a, const ams_dcsr_t *b
 	int dtype, int64_t *row_ptr_c_dev, int64_t *nnz_c_host);
 
 /* Numeric phase: fills col_c / val_c (nnz_c entries, device) for the row
- * pointers of the preceding symbolic call.  Replaces pack_ams_pack +
- * ams_hybacc_numc (src/ams/ams_utils.c:1410-1596,1080-1338). */
+ * pointers of the preceding symbolic call, which must have been made on the
+ * same A and B (same sizes and the same row-pointer and column arrays; the
+ * values may differ), else AMS_ERR_STATE.  Synchronises the stream at the
+ * end and returns AMS_ERR_STATE when a kernel raised an error flag (see
+ * ams_get_error_flags).  Replaces pack_ams_pack + ams_hybacc_numc
+ * (src/ams/ams_utils.c:1410-1596,1080-1338). */
 int ams_spgemm_numeric(ams_ctx_t *ctx, const ams_dcsr_t *a, const ams_dcsr_t *b, int dtype,
 	const int64_t *row_ptr_c_dev, int32_t *col_c_dev, void *val_c_dev);
 
@@ -164,12 +168,27 @@ int ams_batched_run(ams_ctx_t *ctx, const ams_dcsr_t *a_stack, const ams_dcsr_t 
 	int dtype, int64_t *c_row_ptr_dev, int32_t *c_col_dev, void *c_val_dev, int64_t c_cap,
 	int64_t *nnz_dev);
 /* Captures one ams_batched_run into a graph on ctx's stream (which must not
- * be the null stream) and returns the executable graph. */
+ * be the null stream) and returns an opaque graph handle.  The graph's
+ * kernels point into ctx's scratch: ams_graph_launch replays it on ctx only,
+ * and only while that scratch has neither moved (a later call on ctx needing
+ * a larger buffer reallocates it) nor been re-planned (another
+ * ams_batched_plan); otherwise it returns AMS_ERR_STATE and the graph must
+ * be captured again.  Replays and other calls on ctx are ordered on its
+ * stream.  ams_graph_destroy releases the handle. */
 int ams_batched_capture(ams_ctx_t *ctx, const ams_dcsr_t *a_stack, const ams_dcsr_t *b_stack,
 	int dtype, int64_t *c_row_ptr_dev, int32_t *c_col_dev, void *c_val_dev, int64_t c_cap,
 	int64_t *nnz_dev, void **graph_exec);
 int ams_graph_launch(ams_ctx_t *ctx, void *graph_exec);
 int ams_graph_destroy(void *graph_exec);
+
+/* Error flags the kernels of the last run on ctx raised, read after the
+ * stream drains (so it also serves after a graph replay or a batched run,
+ * which do not synchronise): 0x1 LDS-hash row count differs from its
+ * symbolic count, 0x2 ESC row count differs, 0x4 dense chunk count
+ * differs, 0x8 symbolic hash table full, 0x10 numeric hash table full.
+ * Any non-zero value means C is not to be trusted.  No counterpart in the
+ * reference (its kernels cannot detect these). */
+int ams_get_error_flags(ams_ctx_t *ctx, int32_t *flags);
 
 /* Masked-SpGEMM backward of C = A * B with upstream gradient G on pattern(C)
  * (same row pointers and columns as C): dA = (G * B^T) masked to pattern(A),

@@ -116,3 +116,65 @@ def test_batched_heavy_rows(gpu_ctx):
             assert_matches_oracle(c, a, b, exact=False)
     bp.close()
     ctx.close()
+
+
+def test_batched_mixed_widths(gpu_ctx):
+    """A batch that stacks one problem wider than 16384 columns (R-MAT s15,
+    32768 columns) with narrow s8-s10 problems: the narrow problems' light
+    bitmap rows are counted by the wave kernel, which must run although the
+    widest problem is too wide for it; every product against the oracle."""
+    import torch
+    from ams_hip.batched import BatchedProduct
+    from ams_hip.device import Context
+    probs = []
+    for i, (s, ef) in enumerate([(8, 16), (9, 32), (15, 4), (10, 16), (8, 32)]):
+        a = gen.rmat_symmetric(s, ef, seed=700 + i, values="uniform")
+        probs.append((a, a))
+    stream = torch.cuda.Stream()
+    ctx = Context(stream=stream)
+    ctx.set_params(bitmap_ratio=1.0)
+    bp = BatchedProduct(probs, ams=ctx)
+    print(f"[batched] mixed widths s8..s15 cap {bp.cap}", flush=True)
+    for rep in range(2):
+        with torch.cuda.stream(stream):
+            bp.c_val.fill_(float("nan"))
+            bp.run()
+        res = bp.results()
+        assert int(bp.nnz.cpu()[0]) == bp.cap
+        for (a, b), c in zip(probs, res):
+            assert c.sorted_rows()
+            assert_matches_oracle(c, a, b, exact=False)
+    bp.close()
+    ctx.close()
+
+
+def test_stale_graph_rejected(gpu_ctx):
+    """A captured batch replays only while the context's scratch stays put:
+    a larger product on the same context reallocates buffers and a new plan
+    rewrites the problem tables, after which the old graph is refused."""
+    import torch
+    from ams_hip import _lib as L
+    from ams_hip.batched import BatchedProduct
+    from ams_hip.device import Context, DeviceCsr
+    probs = small_batch(6, seed=41)
+    stream = torch.cuda.Stream()
+    ctx = Context(stream=stream)
+    bp = BatchedProduct(probs, ams=ctx)
+    bp.run()
+    bp.results()
+    big = gen.rmat_symmetric(14, 16, seed=5)
+    with torch.cuda.stream(stream):
+        d = DeviceCsr.from_host(big)
+    stream.synchronize()
+    ctx.spgemm_pooled(d, d)
+    with pytest.raises(L.AmsError, match="stale"):
+        bp.run()
+    bp2 = BatchedProduct(probs, ams=ctx)  # re-plans and captures afresh
+    bp2.run()
+    for (a, b), c in zip(probs, bp2.results()):
+        assert_matches_oracle(c, a, b, exact=False)
+    with pytest.raises(L.AmsError, match="stale"):
+        bp.run()
+    bp.close()
+    bp2.close()
+    ctx.close()

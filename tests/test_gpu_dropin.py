@@ -138,3 +138,22 @@ def test_phase_functions(gpu_ctx):
     np.testing.assert_allclose(rats, orats, rtol=1e-12, atol=1e-13)
     L.free_ams_pack(C.byref(pack))
     L.free_ams_bin(C.byref(binr))
+
+
+def test_numeric_needs_same_operands(gpu_ctx):
+    """The numeric phase refuses operands other than those of the symbolic
+    call it follows (same shape but other arrays): its routing and chunk
+    plan would not describe them."""
+    import torch
+    from ams_hip import _lib as L
+    from ams_hip.device import DeviceCsr
+    a = gen.rmat_symmetric(9, 8, seed=1)
+    b = gen.rmat_symmetric(9, 8, seed=2)
+    da, db = DeviceCsr.from_host(a), DeviceCsr.from_host(b)
+    rp, nnz = gpu_ctx.symbolic(da, da)
+    with pytest.raises(L.AmsError, match="same A and B"):
+        gpu_ctx.numeric(db, db, rp, nnz)
+    c = gpu_ctx.numeric(da, da, rp, nnz)
+    torch.cuda.synchronize()
+    assert gpu_ctx.error_flags() == 0
+    assert c.num_nnz == nnz
