@@ -77,6 +77,7 @@ SIGNATURES = {
     "ams_graph_launch": (C.c_int, [P, P]),
     "ams_graph_destroy": (C.c_int, [P]),
     "ams_get_error_flags": (C.c_int, [P, C.POINTER(C.c_int32)]),
+    "ams_spgemm_fixed": (C.c_int, [P, PD, PD, C.c_int, P, P, P, I64, P]),
     "ams_spgemm_masked_bwd": (C.c_int, [P, PD, PD, PD, C.c_int, P, P]),
     "ams_autotune": (C.c_int, [P, PD, PD, C.c_int, C.c_int, C.POINTER(AmsParams), C.POINTER(D)]),
     "ams_verify_result": (C.c_int, [P, PD, C.c_int, C.POINTER(I64), C.POINTER(I64), C.POINTER(D),

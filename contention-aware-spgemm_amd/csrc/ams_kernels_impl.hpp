@@ -1930,6 +1930,10 @@ __global__ void k_copy_i64(const int64_t* src, int64_t* dst) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
 }
 
+__global__ void k_copy_i32(const int32_t* src, int32_t* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
 // Work per route, for reports and per-kernel rooflines:
 //   sums[r]     products of numeric route r (0 hash, 1 ESC, 2 dense)
 //   sums[3 + r] nnz(C) of route r, sums[6 + r] nnz(A) of route r

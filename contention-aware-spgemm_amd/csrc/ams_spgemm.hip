@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "ams_hip.h"
@@ -85,6 +86,34 @@ constexpr int symb_hash_threads() { return S / 16 < 64 ? 64 : (S / 16 > 512 ? 51
 
 }  // namespace
 
+// Fixed scratch of ams_spgemm_fixed, for callers that cannot allocate: one
+// array per GPU in device memory, reserved when the library's code object
+// loads on that GPU and held until the process ends.  Host code never uses
+// this name as a device address (on the host it is a stand-in symbol): it
+// takes the GPU's address from hipGetSymbolAddress once per device, with the
+// device current, and uses that address only.
+constexpr size_t kFixedScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_addr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of the fixed scratch on `device`, which must be current.
+static hipError_t fixed_scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
+}
+
 struct ams_ctx {
   hipStream_t stream = nullptr;
   int device = 0;
@@ -128,6 +157,8 @@ struct ams_ctx {
   // generation it was captured at is current.
   uint64_t gen = 1;
   bool capturing = false;  // inside ams_batched_capture: buffers must not move
+  bool fixed = false;      // scratch carved from the fixed device array: nothing may grow
+  unsigned char* fixed_base = nullptr;  // device address of this GPU's fixed scratch
 };
 
 // Handle of a captured batched run: the executable graph, the context it
@@ -165,6 +196,8 @@ static int fail(ams_ctx_t* ctx, int code, const char* fmt, ...) {
 static int grow(ams_ctx_t* ctx, Buf& b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (b.cap >= bytes) return AMS_OK;
+  if (ctx->fixed)
+    return fail(ctx, AMS_ERR_LIMIT, "a scratch buffer of %zu bytes exceeds its share of the fixed scratch", bytes);
   if (ctx->capturing)
     return fail(ctx, AMS_ERR_STATE, "a scratch buffer of %zu bytes would move during graph capture; "
                 "call ams_batched_plan for these inputs first", bytes);
@@ -572,7 +605,9 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
       const size_t dyn = (size_t)(Wb / 32) * 4 + (size_t)(Wb / W + 1) * 4;
       // heavy rows (more than max(2^20, P/2048) products) are split into
       // pieces when their global bitmaps fit 512 MiB
-      const int64_t gwords = heavy_words(pr);
+      // (not in the fixed scratch, which has no room for the global bitmaps:
+      // heavy rows are then counted whole)
+      const int64_t gwords = ctx->fixed ? 0 : heavy_words(pr);
       const bool split = gwords > 0;
       const uint8_t* hflag = nullptr;
       if (split) {
@@ -941,6 +976,16 @@ int ams_ctx_create(ams_ctx_t** out, void* stream) {
     ams_ctx_destroy(ctx);
     return AMS_ERR_ALLOC;
   }
+  // this GPU's fixed scratch (ams_spgemm_fixed) and the context's own control
+  // block, so that ams_spgemm_fixed reports its flags without allocating
+  void* fs = nullptr;
+  if (fixed_scratch_address(ctx->device, &fs) != hipSuccess || grow(ctx, ctx->ctrl, sizeof(Ctrl)) != AMS_OK ||
+      fill_i32(ctx, (int32_t*)ctx->ctrl.p, sizeof(Ctrl) / 4, 0) != AMS_OK) {
+    (void)hipGetLastError();
+    ams_ctx_destroy(ctx);
+    return AMS_ERR_HIP;
+  }
+  ctx->fixed_base = (unsigned char*)fs;
   *out = ctx;
   return AMS_OK;
 }
@@ -1299,6 +1344,103 @@ int ams_batched_run(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, in
   hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, ctx->stream, (const int64_t*)(c_rp + a->num_row), nnz_dev);
   HIPCHK(hipGetLastError());
   return AMS_OK;
+}
+
+// The context's scratch buffers, swapped for slices of the fixed scratch for
+// the duration of one ams_spgemm_fixed call and restored afterwards.
+struct FixedSwap {
+  ams_ctx_t* ctx;
+  static constexpr int kN = 27;
+  Buf* bufs[kN];
+  Buf saved[kN];
+  explicit FixedSwap(ams_ctx_t* c) : ctx(c) {
+    Buf* all[kN] = {&c->row_flp, &c->sbin, &c->nbin, &c->lsym, &c->lnum, &c->ctrl, &c->coff,
+                    &c->bptr, &c->bcol, &c->bval, &c->bperm, &c->tsum, &c->pool_rp, &c->pool_col,
+                    &c->pool_val, &c->probs, &c->scratch_a, &c->dflag, &c->dtask, &c->ccnt,
+                    &c->prow, &c->pbrow, &c->lrows, &c->hflag, &c->hrow, &c->hpiece, &c->gbm};
+    for (int i = 0; i < kN; ++i) {
+      bufs[i] = all[i];
+      saved[i] = *all[i];
+      *all[i] = Buf{};  // unused buffers stay empty: any use fails instead of touching the context's memory
+    }
+    c->fixed = true;
+  }
+  ~FixedSwap() {
+    for (int i = 0; i < kN; ++i) *bufs[i] = saved[i];
+    ctx->fixed = false;
+    // the plan of this call lived in the fixed scratch
+    ctx->plan_valid = false;
+    ctx->plan_counts = false;
+    ctx->bblk_ready = false;
+  }
+};
+
+int ams_spgemm_fixed(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int64_t* c_rp,
+                     int32_t* c_col, void* c_val, int64_t cap, int64_t* nnz_dev) {
+  CHK(check_pair(ctx, a, b, dtype));
+  if (!c_rp || !nnz_dev || cap < 0 || (cap > 0 && (!c_col || !c_val)))
+    return fail(ctx, AMS_ERR_ARG, "NULL output buffer or negative capacity");
+  if (!ctx->fixed_base) return fail(ctx, AMS_ERR_STATE, "no fixed scratch address on this context's GPU");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != ctx->device) return fail(ctx, AMS_ERR_STATE, "the context belongs to GPU %d, GPU %d is current",
+                                      ctx->device, dev);
+  if (ctx->capturing) return fail(ctx, AMS_ERR_STATE, "a batched capture is in progress on this context");
+  const Probs pr = single_probs(b);
+  int W, R, Wb;
+  dense_geometry(ctx, dtype, pr.max_width, W, R, Wb);
+  const int64_t m1 = std::max<int64_t>(a->num_row, 1), mB = b->num_row;
+  const int64_t nnzB1 = std::max<int64_t>(b->num_nnz, 1);
+  const size_t vb = dtype == AMS_F64 ? 8 : 4;
+  // every scan of the unsynchronised run: row pointers of C, the blocked
+  // copy's chunk pointers, the dense morsel flags
+  const int64_t nscan = std::max<int64_t>(std::max<int64_t>(m1 + 1, (int64_t)R * (mB + 1)), m1 * R + 1);
+  // the buffers the unsynchronised run grows (every row may be a wide row)
+  struct Part { Buf* buf; size_t bytes; };
+  const Part parts[] = {
+      {&ctx->ctrl, sizeof(Ctrl)},
+      {&ctx->row_flp, sizeof(int64_t) * (size_t)m1},
+      {&ctx->sbin, (size_t)m1},
+      {&ctx->nbin, (size_t)m1},
+      {&ctx->lsym, sizeof(int32_t) * (size_t)m1},
+      {&ctx->lnum, sizeof(int32_t) * (size_t)m1},
+      {&ctx->coff, sizeof(int32_t) * (size_t)m1 * (R + 1)},
+      {&ctx->ccnt, ctx->prm.sym_blocked ? sizeof(int32_t) * (size_t)m1 * R : 0},
+      {&ctx->bptr, sizeof(int64_t) * (size_t)R * (mB + 1)},
+      {&ctx->bcol, sizeof(uint16_t) * (size_t)nnzB1},
+      {&ctx->bval, vb * (size_t)nnzB1},
+      {&ctx->lrows, sizeof(int32_t) * (size_t)std::max<int64_t>(mB, 1)},
+      {&ctx->dflag, sizeof(int64_t) * (size_t)(m1 * R + 1)},
+      {&ctx->dtask, sizeof(DenseTask) * (size_t)(m1 * R)},
+      {&ctx->tsum, sizeof(int64_t) * (size_t)((nscan + kScanTile - 1) / kScanTile)},
+  };
+  size_t need = 0;
+  for (const Part& q : parts) need += (q.bytes + 255) & ~(size_t)255;
+  if (need > kFixedScratchBytes)
+    return fail(ctx, AMS_ERR_LIMIT, "this product needs %zu bytes of scratch; the fixed scratch holds %zu "
+                "(use ams_spgemm or ams_spgemm_symbolic/numeric, which allocate)", need, kFixedScratchBytes);
+  Ctrl* own = dctrl(ctx);  // the context's control block receives the flags
+  int rc;
+  {
+    FixedSwap swap(ctx);
+    size_t off = 0;
+    for (const Part& q : parts) {
+      if (q.bytes == 0) continue;
+      *q.buf = Buf{ctx->fixed_base + off, q.bytes};
+      off += (q.bytes + 255) & ~(size_t)255;
+    }
+    rc = run_symbolic(ctx, a, b, dtype, pr, c_rp, false, nullptr);
+    if (rc == AMS_OK) rc = run_numeric(ctx, a, b, dtype, c_rp, c_col, c_val, cap);
+    if (rc == AMS_OK) {
+      hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, ctx->stream, (const int64_t*)(c_rp + a->num_row),
+                         nnz_dev);
+      hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, ctx->stream, (const int32_t*)&dctrl(ctx)->err,
+                         &own->err);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail(ctx, AMS_ERR_HIP, "launch: %s", hipGetErrorString(e));
+    }
+  }
+  return rc;
 }
 
 int ams_batched_capture(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, int64_t* c_rp,

@@ -181,6 +181,27 @@ int ams_batched_capture(ams_ctx_t *ctx, const ams_dcsr_t *a_stack, const ams_dcs
 int ams_graph_launch(ams_ctx_t *ctx, void *graph_exec);
 int ams_graph_destroy(void *graph_exec);
 
+/* One-shot product that allocates nothing, for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * It runs the unsynchronised pipeline of ams_batched_run for one product
+ * with every scratch buffer carved from a fixed array of 64 MiB per GPU
+ * (reserved when the library loads on that GPU, held until the process
+ * ends; G500 R-MAT scale 16 edge factor 16 fits), so a product whose
+ * scratch would exceed it fails with AMS_ERR_LIMIT and does nothing.  It
+ * neither allocates nor synchronises: c_row_ptr (num_row(A) + 1) receives
+ * C's row pointers and *nnz_dev (device int64) nnz(C); c_col / c_val are
+ * caller buffers of c_cap entries, and a C of more than c_cap entries
+ * writes none (read *nnz_dev, then call again with a larger buffer).
+ * ams_get_error_flags(ctx) reads the call's kernel error flags.  Heavy
+ * rows are counted whole (no split into pieces).  All calls on one GPU
+ * share its fixed array and leave their contents in it: calls that overlap
+ * on one GPU (other streams or contexts) race, so order them on one stream.
+ * The call does not change ctx's own scratch; it invalidates ctx's
+ * symbolic plan (an ams_spgemm_numeric must follow a new symbolic call). */
+int ams_spgemm_fixed(ams_ctx_t *ctx, const ams_dcsr_t *a, const ams_dcsr_t *b, int dtype,
+	int64_t *c_row_ptr_dev, int32_t *c_col_dev, void *c_val_dev, int64_t c_cap, int64_t *nnz_dev);
+
 /* Error flags the kernels of the last run on ctx raised, read after the
  * stream drains (so it also serves after a graph replay or a batched run,
  * which do not synchronise): 0x1 LDS-hash row count differs from its
