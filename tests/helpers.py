@@ -54,3 +54,40 @@ def random_csr(m, n, density, seed, values="uniform", sort=True, dtype=np.float6
             cols[s:e] = cols[s:e][perm]
             vals[s:e] = vals[s:e][perm]
     return Csr(m, n, rp, cols, vals.astype(dtype))
+
+
+def oracle_layout(m: Csr) -> Csr:
+    """m in the oracle's int64 / float64 layout (convert large operands once)."""
+    return m.astype(np.int64, np.float64)
+
+
+def oracle_rows(a: Csr, b64: Csr, rows, mode="hybacc"):
+    """The oracle's product of the listed rows of A with B (b64 already in
+    oracle_layout), in one call; returns (cols int64, vals float64) per row."""
+    from oracle import oracle as orc
+    rows = np.asarray(rows, np.int64)
+    rp_a = np.asarray(a.row_ptrs, np.int64)
+    cnt = rp_a[rows + 1] - rp_a[rows]
+    rp = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum(cnt, out=rp[1:])
+    idx = np.concatenate([np.arange(rp_a[r], rp_a[r + 1]) for r in rows]) if len(rows) else np.zeros(0, np.int64)
+    sub = Csr(len(rows), a.num_col, rp, np.asarray(a.cols)[idx].astype(np.int64),
+              np.asarray(a.rats)[idx].astype(np.float64))
+    orp, ocols, orats, _ = orc.spgemm(sub, b64, mode)
+    return [(ocols[orp[i]:orp[i + 1]], orats[orp[i]:orp[i + 1]]) for i in range(len(rows))]
+
+
+def device_row(c, rp_host, r):
+    """Columns (int64) and values of row r of a device CSR C."""
+    s, e = int(rp_host[r]), int(rp_host[r + 1])
+    return c.col[s:e].cpu().numpy().astype(np.int64), c.val[s:e].cpu().numpy()
+
+
+def sample_rows(row_flp, count_heavy, count_random, seed):
+    """The count_heavy rows with the most products and count_random others
+    drawn at random (seeded), ascending and distinct."""
+    flp = np.asarray(row_flp)
+    rng = np.random.default_rng(seed)
+    heavy = np.argsort(flp, kind="stable")[-count_heavy:]
+    rand = rng.choice(len(flp), min(count_random, len(flp)), replace=False)
+    return np.unique(np.concatenate([heavy, rand]))

@@ -291,3 +291,49 @@ def test_fp32_chunk_width(gpu_ctx):
     np.testing.assert_array_equal(c32.row_ptrs, c64.row_ptrs)
     np.testing.assert_array_equal(c32.cols, c64.cols)
     np.testing.assert_array_equal(c32.rats, c64.rats.astype(np.float32))
+
+
+ESC_CASES = [("rmat_s8_ef4", lambda: gen.rmat_symmetric(8, 4, seed=101, values="uniform")),
+             ("rmat_s10_ef8", lambda: gen.rmat_symmetric(10, 8, seed=102, values="uniform")),
+             ("er_s11_ef8", lambda: gen.rmat_symmetric(11, 8, gen.ER, seed=103, values="uniform")),
+             ("rect_unsorted", None)]
+
+
+@pytest.mark.parametrize("esc_sort", [0, 1])
+@pytest.mark.parametrize("name,make", ESC_CASES, ids=[c[0] for c in ESC_CASES])
+def test_esc_random_values_bit_exact(gpu_ctx, name, make, esc_sort):
+    """Random FP64 values through the ESC route: every row of at most esc_max
+    products is summed left to right in A-row order from separately rounded
+    products, as the reference's compress does (src/ams/ams_utils.c:1062-1067),
+    so its values equal the oracle's bit for bit; a fused multiply-add or any
+    other order would change the last bits of most entries.  Longer rows (hash
+    or dense routes) are held to the 1e-12 * (|A||B|)_ij bound."""
+    import torch
+    from ams_hip.device import DeviceCsr
+    from helpers import abs_csr, oracle_product
+    if make is None:
+        a, b = random_csr(90, 400, 0.06, seed=11), random_csr(400, 3000, 0.03, seed=12, sort=False)
+    else:
+        a = make()
+        b = a
+    esc_max = 4096
+    # ESC for every row of at most esc_max products: no bitmap symbolic below
+    # 5461 products and no dense chunks for such rows
+    c = run(gpu_ctx, a, b, dict(mode=2, esc_max=esc_max, esc_sort=esc_sort, sym_hash_max=5461,
+                                num_hash_max=5461, bitmap_ratio=1e9, dense_density=2.0))
+    routed_esc = gpu_ctx.stats()["rows_num_esc"]
+    flp, _ = gpu_ctx.row_flops(DeviceCsr.from_host(a), DeviceCsr.from_host(b))
+    flp = flp.cpu().numpy()
+    torch.cuda.synchronize()
+    rp, cols, vals = oracle_product(a, b)
+    np.testing.assert_array_equal(np.asarray(c.row_ptrs, np.int64), rp)
+    np.testing.assert_array_equal(np.asarray(c.cols, np.int64), cols)
+    esc_row = (flp <= esc_max) & (np.diff(rp) > 0)
+    assert routed_esc == int(esc_row.sum()), "every row of at most esc_max products took ESC"
+    esc_rows = np.repeat(esc_row, np.diff(rp))
+    assert esc_rows.mean() > 0.5, "most entries must come from ESC rows"
+    np.testing.assert_array_equal(c.rats[esc_rows], vals[esc_rows])
+    _, _, bound = oracle_product(abs_csr(a), abs_csr(b))
+    assert np.all(np.abs(c.rats - vals) <= 1e-12 * bound + 1e-300)
+    print(f"[parity] {name} esc_sort={esc_sort}: {int(esc_rows.sum())} of {len(vals)} entries from ESC rows "
+          f"bit-exact", flush=True)
