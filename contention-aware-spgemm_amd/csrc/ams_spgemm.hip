@@ -143,6 +143,7 @@ struct ams_ctx {
   std::vector<Rec> recs;
   double acc_ms[AMS_PH_COUNT] = {};
   int64_t acc_n[AMS_PH_COUNT] = {};
+  ams_params_t plan_prm{};       // parameters the plan was made with
   // operands of the plan: the numeric phase must see the same matrices
   int64_t plan_nnzA = 0, plan_nB = 0;
   const void *plan_arp = nullptr, *plan_acol = nullptr, *plan_brp = nullptr, *plan_bcol = nullptr;
@@ -500,6 +501,73 @@ static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const 
              : launch_symb_bitmap_q<T, false, 0>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords, wave_rows);
 }
 
+// Numeric route of every row (k_classify_num) from the symbolic phase's
+// bins and C's row pointers, rows grouped by route; with sync, the route
+// counts, nnz(C) and the statistics come back to the host.  The control
+// block's numeric counters and route sums must be zero.
+static int route_numeric(ams_ctx_t* ctx, const Csr& A, const Probs& pr, const int64_t* rp_c, int R, bool sync,
+                         int64_t* nnz) {
+  const int64_t m = A.m;
+  Ctrl* c = dctrl(ctx);
+  const NumParams np{ctx->prm.num_hash_max, ctx->prm.esc_max, ctx->prm.acc_thres, ctx->prm.dense_density,
+               ctx->prm.mode};
+  if (m > 0) {
+    Phase ph(ctx, AMS_PH_CLASSIFY, sync);
+    const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
+    hipLaunchKernelGGL(k_classify_num, dim3(gc), dim3(256), 0, ctx->stream, m,
+                       (const int64_t*)ctx->row_flp.p, (const int64_t*)rp_c, (const uint8_t*)ctx->sbin.p, pr,
+                       np, (uint8_t*)ctx->nbin.p, c->cnt_num);
+    hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->nbin.p,
+                       kNumBins, (const int32_t*)c->cnt_num, c->cur_num, (int32_t*)ctx->lnum.p);
+    if (sync)
+      hipLaunchKernelGGL(k_route_sums, dim3(gc), dim3(256), 0, ctx->stream, m, (const int64_t*)ctx->row_flp.p,
+                         (const int64_t*)rp_c, A.rp, (const uint8_t*)ctx->sbin.p, (const uint8_t*)ctx->nbin.p,
+                         c->sums);
+    HIPCHK(hipGetLastError());
+  }
+  if (sync) {
+    HIPCHK(hipMemcpyAsync(ctx->h_ctrl, c, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_pin, rp_c + m, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(ctx->cnt_num, ctx->h_ctrl->cnt_num, sizeof(ctx->cnt_num));
+    if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "symbolic kernels raised error flags 0x%x", ctx->h_ctrl->err);
+    if (nnz) *nnz = ctx->h_pin[0];
+    ams_stats_t& s = ctx->stats;
+    s.rows_total = m;
+    s.rows_sym_bitmap = ctx->cnt_sym[SB_BITMAP];
+    s.rows_sym_hash = 0;
+    for (int i = 0; i < kSymHashBins; ++i) s.rows_sym_hash += ctx->cnt_sym[i];
+    s.rows_empty = m - s.rows_sym_hash - s.rows_sym_bitmap;
+    s.rows_num_hash = s.rows_num_esc = 0;
+    for (int i = 0; i < kNumHashBins; ++i) s.rows_num_hash += ctx->cnt_num[i];
+    for (int i = 0; i < kNumEscBins; ++i) s.rows_num_esc += ctx->cnt_num[kNumHashBins + i];
+    s.rows_num_dense = ctx->cnt_num[NB_DENSE];
+    s.dense_chunks = R;
+    s.nnz_c = ctx->h_pin[0];
+    for (int k = 0; k < 3; ++k) {
+      s.flops_route[k] = (int64_t)ctx->h_ctrl->sums[k];
+      s.nnzc_route[k] = (int64_t)ctx->h_ctrl->sums[3 + k];
+      s.nnza_route[k] = (int64_t)ctx->h_ctrl->sums[6 + k];
+    }
+    for (int k = 0; k < 2; ++k) {
+      s.flops_sym[k] = (int64_t)ctx->h_ctrl->sums[9 + k];
+      s.nnza_sym[k] = (int64_t)ctx->h_ctrl->sums[11 + k];
+    }
+  }
+  return AMS_OK;
+}
+
+// The routing parameters of the numeric phase changed since the symbolic
+// phase (same geometry): route again without counting again.
+static int reroute_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const int64_t* rp_c) {
+  Ctrl* c = dctrl(ctx);
+  CHK(fill_i32(ctx, c->cnt_num, 2 * 32, 0));  // cnt_num and cur_num
+  CHK(fill_i64(ctx, (int64_t*)c->sums, 13, 0));
+  CHK(route_numeric(ctx, to_csr(a), ctx->plan_pr, rp_c, ctx->plan_R, true, nullptr));
+  ctx->plan_prm = ctx->prm;
+  return AMS_OK;
+}
+
 static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype,
                         const Probs& pr, int64_t* rp_c, bool sync, int64_t* nnz) {
   const Csr A = to_csr(a), B = to_csr(b);
@@ -660,51 +728,8 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     CHK(scan_i64(ctx, rp_c, rp_c, m + 1));
   }
 
-  NumParams np{ctx->prm.num_hash_max, ctx->prm.esc_max, ctx->prm.acc_thres, ctx->prm.dense_density,
-               ctx->prm.mode};
-  if (m > 0) {
-    Phase ph(ctx, AMS_PH_CLASSIFY, sync);
-    const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
-    hipLaunchKernelGGL(k_classify_num, dim3(gc), dim3(256), 0, ctx->stream, m,
-                       (const int64_t*)ctx->row_flp.p, (const int64_t*)rp_c, (const uint8_t*)ctx->sbin.p, pr,
-                       np, (uint8_t*)ctx->nbin.p, c->cnt_num);
-    hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->nbin.p,
-                       kNumBins, (const int32_t*)c->cnt_num, c->cur_num, (int32_t*)ctx->lnum.p);
-    if (sync)
-      hipLaunchKernelGGL(k_route_sums, dim3(gc), dim3(256), 0, ctx->stream, m, (const int64_t*)ctx->row_flp.p,
-                         (const int64_t*)rp_c, A.rp, (const uint8_t*)ctx->sbin.p, (const uint8_t*)ctx->nbin.p,
-                         c->sums);
-    HIPCHK(hipGetLastError());
-  }
-  if (sync) {
-    HIPCHK(hipMemcpyAsync(ctx->h_ctrl, c, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->h_pin, rp_c + m, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::memcpy(ctx->cnt_num, ctx->h_ctrl->cnt_num, sizeof(ctx->cnt_num));
-    if (ctx->h_ctrl->err) return fail(ctx, AMS_ERR_STATE, "symbolic kernels raised error flags 0x%x", ctx->h_ctrl->err);
-    if (nnz) *nnz = ctx->h_pin[0];
-    ams_stats_t& s = ctx->stats;
-    s.rows_total = m;
-    s.rows_sym_bitmap = ctx->cnt_sym[SB_BITMAP];
-    s.rows_sym_hash = 0;
-    for (int i = 0; i < kSymHashBins; ++i) s.rows_sym_hash += ctx->cnt_sym[i];
-    s.rows_empty = m - s.rows_sym_hash - s.rows_sym_bitmap;
-    s.rows_num_hash = s.rows_num_esc = 0;
-    for (int i = 0; i < kNumHashBins; ++i) s.rows_num_hash += ctx->cnt_num[i];
-    for (int i = 0; i < kNumEscBins; ++i) s.rows_num_esc += ctx->cnt_num[kNumHashBins + i];
-    s.rows_num_dense = ctx->cnt_num[NB_DENSE];
-    s.dense_chunks = R;
-    s.nnz_c = ctx->h_pin[0];
-    for (int k = 0; k < 3; ++k) {
-      s.flops_route[k] = (int64_t)ctx->h_ctrl->sums[k];
-      s.nnzc_route[k] = (int64_t)ctx->h_ctrl->sums[3 + k];
-      s.nnza_route[k] = (int64_t)ctx->h_ctrl->sums[6 + k];
-    }
-    for (int k = 0; k < 2; ++k) {
-      s.flops_sym[k] = (int64_t)ctx->h_ctrl->sums[9 + k];
-      s.nnza_sym[k] = (int64_t)ctx->h_ctrl->sums[11 + k];
-    }
-  }
+  CHK(route_numeric(ctx, A, pr, rp_c, R, sync, nnz));
+  ctx->plan_prm = ctx->prm;
   ctx->plan_valid = true;
   return AMS_OK;
 }
@@ -1105,6 +1130,13 @@ int ams_spgemm_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b,
     return fail(ctx, AMS_ERR_STATE, "numeric needs the symbolic phase of the same A and B (same arrays) first");
   if (!rp_c || (ctx->stats.nnz_c > 0 && (!col_c || !val_c)))
     return fail(ctx, AMS_ERR_ARG, "output buffers are NULL");
+  const ams_params_t &p0 = ctx->plan_prm, &p1 = ctx->prm;
+  if (p0.sym_hash_max != p1.sym_hash_max || p0.dense_width != p1.dense_width || p0.bitmap_bits != p1.bitmap_bits ||
+      p0.bitmap_ratio != p1.bitmap_ratio || p0.sym_blocked != p1.sym_blocked)
+    return fail(ctx, AMS_ERR_STATE, "symbolic parameters changed since the symbolic phase; run it again");
+  if (p0.num_hash_max != p1.num_hash_max || p0.esc_max != p1.esc_max || p0.acc_thres != p1.acc_thres ||
+      p0.dense_density != p1.dense_density || p0.mode != p1.mode)
+    CHK(reroute_numeric(ctx, a, rp_c));
   CHK(run_numeric(ctx, a, b, dtype, rp_c, col_c, val_c, INT64_MAX));
   return check_flags(ctx, "numeric");
 }

@@ -6,8 +6,9 @@
  * src/mm/mm_utils.h, src/ams/ams_utils.h) links against libams_hip.so
  * instead of libAms/libMm/libInc and keeps its calls.  Types keep the
  * reference's byte layout (int64 indices, double values).  Inputs are host
- * arrays: every call uploads what it needs, runs the device pipeline on one
- * GPU and downloads its outputs.  Errors print a message and exit, as the
+ * arrays: a call uploads what it needs (once per driver sequence, see
+ * ams_dropin_counters), runs the device pipeline on one GPU and downloads
+ * its outputs.  Errors print a message and exit, as the
  * reference does (src/inc/utils.c:205-216).  Not reentrant: one process-wide
  * device context, like the reference's globals (src/ams/ams_utils.c:9,20,31-32).
  *
@@ -136,12 +137,32 @@ extern my_rat_t ams_ht_load;
 my_cnt_t ams_ht_slots_for(my_cnt_t nz);
 void ams_hash_symb(csr_t *csr_a, csr_t *csr_b, my_cnt_t *ht_check, ams_bin_t *ams_bin);
 void ams_hash_symb_dnr(csr_t *csr_a, csr_t *csr_b, my_cnt_t *ht_check, ams_bin_t *ams_bin);
+/* src/ams/ams_utils.h:88 (src/ams/ams_utils.c:1754-1846): the OpenMP form of
+ * the dynamic-range symbolic; the same row counts. */
+void ams_hash_symb_omp_dnr(csr_t *csr_a, csr_t *csr_b, my_cnt_t *ht_check, ams_bin_t *ams_bin);
+/* src/ams/ams_utils.h:92-94 (src/ams/ams_utils.c:1851-2046): the morsel table
+ * over nonzeros of csr (ps_row_coef: scaled prefix of nonzeros per row). */
+my_cnt_t pack_ams_pack_mmmv(csr_t *csr, ams_pack_t *ams_pack, my_cnt_t *ps_row_coef, my_cnt_t *nnz_per_row,
+	my_cnt_t coef, numa_cfg_t *numa_cfg);
 void ams_hash_numc(csr_t *csr_a, csr_t *csr_b, csr_t *csr_c, my_cnt_t *ht_check, my_rat_t *ht_value,
 	ams_bin_t *ams_bin);
 void ams_hsersc_numc(csr_t *csr_a, csr_t *csr_b, csr_t *csr_c, keyval_t *keyval_org, keyval_t *aux,
 	my_cnt_t max_row_flp, ams_bin_t *ams_bin);
 void ams_hybacc_numc(csr_t *csr_a, csr_t *csr_b, csr_t *csr_c, keyval_t *keyval_org, keyval_t *aux,
 	my_cnt_t max_row_flp, my_cnt_t *ht_check, my_rat_t *ht_value, ams_bin_t *ams_bin);
+
+/* Device copies of a driver sequence.  Between init_ams_bin and free_ams_bin
+ * of one ams_bin_t (the reference's driver sequence,
+ * src/ams_hybaccspgemm.c:53-199) the library keeps A and B on the GPU and
+ * reuses the symbolic phase's plan in the numeric phase, so each input is
+ * uploaded once and the symbolic phase runs once; A and B must not change
+ * during the sequence.  Outside a sequence every call uploads and frees.
+ * No counterpart in the reference: ams_dropin_counters reports the matrices
+ * uploaded and the device symbolic phases run since the process started,
+ * ams_dropin_release drops the kept copies (for a caller that changes A or
+ * B in place inside a sequence). */
+void ams_dropin_counters(my_cnt_t *uploads, my_cnt_t *symbolic_runs);
+void ams_dropin_release(void);
 
 /* Morsel parameters of pack_ams_pack, compile-time in the reference
  * (cmake/Options.cmake:104-128; values of scripts/ams_run_common.sh:26 and

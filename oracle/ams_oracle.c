@@ -13,6 +13,7 @@
  *   orc_hash_symb_dnr      src/ams/ams_utils.c:1600-1752     (ams_hash_symb_dnr)
  *   orc_pack               src/ams/ams_utils.c:1410-1596     (pack_ams_pack)
  *                          + src/mm/mm_utils.c:57-65         (compare_keyvals_3)
+ *   orc_pack_mmmv          src/ams/ams_utils.c:1851-2046     (pack_ams_pack_mmmv)
  *   orc_radix_sort_keyval  src/mm/mm_utils.c:1273-1375       (radix_sort_keyval_toggling_unrolling)
  *   hash_row               src/ams/ams_utils.c:956-1019      (hybacc_hash_krnl)
  *   esc_row                src/ams/ams_utils.c:1023-1075     (hybacc_hsersc_krnl)
@@ -235,6 +236,55 @@ int64_t orc_pack(const int64_t *ps_row_nz, const int64_t *row_flp, const int64_t
 	pack->high_tail_idx = (int64_t)((double)(total - invalid) * p->ams_prt + (double)invalid);
 	pack->low_tail_idx = pack->high_tail_idx - 1;
 	return (int64_t)max_flp;
+}
+
+/* src/ams/ams_utils.c:1851-2046 (AMS_MAXFLP off): morsels over the scaled
+ * nonzero prefix ps_row_coef, total = nnz * coef / T; a valid morsel's sort
+ * value is its mean nonzeros per row, (row_ptrs[e] - row_ptrs[s]) / (e - s),
+ * and its b_roff_maxc value the largest nnz_per_row of its rows.  The rest
+ * (empties, sort, regions) as orc_pack; the same total = 0 guard. */
+int64_t orc_pack_mmmv(const int64_t *row_ptrs, int64_t num_row, int64_t num_nnz, const int64_t *ps_row_coef,
+	const int64_t *nnz_per_row, int64_t coef, const orc_params_t *p, orc_pack_t *pack) {
+	int64_t t = p->ams_min_num;
+	if (p->ams_size / 8 < t) t = p->ams_size / 8;
+	if (t < 1) t = 1;
+	int64_t total = num_nnz * coef / t;
+	if (total < 1) total = 1;  /* guard: the reference exits here */
+	pack->total_cnt = total;
+	pack->b_arr = (orc_keyval_t *)xmalloc(sizeof(orc_keyval_t) * (size_t)total);
+	pack->b_roff_maxc = (orc_keyval_t *)xmalloc(sizeof(orc_keyval_t) * (size_t)(total + 1));
+	pack->b_roff_maxc[0].key = 0;
+	pack->b_roff_maxc[0].val = 0.0;
+	for (int64_t k = 0; k < total; k++) {
+		int64_t *lb = orc_lower_bound((int64_t *)ps_row_coef, (int64_t *)ps_row_coef + num_row + 1,
+			t * (k + 1));
+		pack->b_roff_maxc[k + 1].key = lb - ps_row_coef;
+		pack->b_roff_maxc[k + 1].val = 0.0;
+		pack->b_arr[k].key = k;
+	}
+	pack->b_roff_maxc[total].key = num_row;
+	int64_t invalid = 0;
+	double max_nz = 0.0;
+	for (int64_t k = 0; k < total; k++) {
+		int64_t s = pack->b_roff_maxc[k].key, e = pack->b_roff_maxc[k + 1].key;
+		if (e - s > 0) {
+			pack->b_arr[k].val = (double)(row_ptrs[e] - row_ptrs[s]) / (double)(e - s);
+			int64_t mx = nnz_per_row[s];
+			for (int64_t r = s + 1; r < e; r++) if (nnz_per_row[r] > mx) mx = nnz_per_row[r];
+			pack->b_roff_maxc[k].val = (double)mx;
+			if ((double)mx > max_nz) max_nz = (double)mx;
+		} else {
+			pack->b_roff_maxc[k].val = 0.0;
+			pack->b_arr[k].val = 0.0;
+			invalid++;
+		}
+	}
+	qsort(pack->b_arr, (size_t)total, sizeof(orc_keyval_t), cmp_val_key);
+	pack->low_head_idx = invalid;
+	pack->high_head_idx = total - 1;
+	pack->high_tail_idx = (int64_t)((double)(total - invalid) * p->ams_prt + (double)invalid);
+	pack->low_tail_idx = pack->high_tail_idx - 1;
+	return (int64_t)max_nz;
 }
 
 void orc_free_pack(orc_pack_t *pack) {

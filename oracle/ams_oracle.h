@@ -85,6 +85,8 @@ void orc_hash_symb_dnr(const orc_csr_t *a, const orc_csr_t *b, const int64_t *ro
 	int64_t *row_nz, const orc_params_t *p, int thread_num);
 int64_t orc_pack(const int64_t *ps_row_nz, const int64_t *row_flp, const int64_t *ps_row_flp,
 	int64_t num_row, const orc_params_t *p, orc_pack_t *pack);
+int64_t orc_pack_mmmv(const int64_t *row_ptrs, int64_t num_row, int64_t num_nnz, const int64_t *ps_row_coef,
+	const int64_t *nnz_per_row, int64_t coef, const orc_params_t *p, orc_pack_t *pack);
 void orc_free_pack(orc_pack_t *pack);
 orc_keyval_t *orc_radix_sort_keyval(orc_keyval_t *arr, orc_keyval_t *aux, int64_t n);
 void orc_numc(const orc_csr_t *a, const orc_csr_t *b, orc_csr_t *c, const int64_t *row_flp,

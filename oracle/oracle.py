@@ -78,6 +78,9 @@ def lib():
         L.orc_pack.argtypes = [P64, P64, P64, C.c_int64, C.POINTER(Params), C.POINTER(_Pack)]
         L.orc_pack.restype = C.c_int64
         L.orc_free_pack.argtypes = [C.POINTER(_Pack)]
+        L.orc_pack_mmmv.argtypes = [P64, C.c_int64, C.c_int64, P64, P64, C.c_int64, C.POINTER(Params),
+                                    C.POINTER(_Pack)]
+        L.orc_pack_mmmv.restype = C.c_int64
         L.orc_radix_sort_keyval.argtypes = [C.POINTER(_KeyVal), C.POINTER(_KeyVal), C.c_int64]
         L.orc_radix_sort_keyval.restype = C.POINTER(_KeyVal)
         L.orc_spgemm.argtypes = [C.POINTER(_Csr), C.POINTER(_Csr), C.POINTER(_Csr),
@@ -174,6 +177,25 @@ def pack(ps_row_nz, row_flp, p=None):
     ps_flp = scan(flp, len(flp) + 1)
     pk = _Pack()
     mx = lib().orc_pack(_p64(ps), _p64(flp), _p64(ps_flp), len(flp), C.byref(p), C.byref(pk))
+    n = pk.total_cnt
+    roff = [(pk.b_roff_maxc[i].key, pk.b_roff_maxc[i].val) for i in range(n + 1)]
+    arr = [(pk.b_arr[i].key, pk.b_arr[i].val) for i in range(n)]
+    idx = dict(total_cnt=n, high_head_idx=pk.high_head_idx, high_tail_idx=pk.high_tail_idx,
+               low_head_idx=pk.low_head_idx, low_tail_idx=pk.low_tail_idx)
+    lib().orc_free_pack(C.byref(pk))
+    return roff, arr, idx, mx
+
+
+def pack_mmmv(row_ptrs, ps_row_coef, nnz_per_row, coef, p=None):
+    """pack_ams_pack_mmmv of src/ams/ams_utils.c:1851-2046 for a CSR with the
+    given row pointers.  Returns what pack() returns."""
+    p = p or params()
+    rp = np.ascontiguousarray(row_ptrs, np.int64)
+    ps = np.ascontiguousarray(ps_row_coef, np.int64)
+    nz = np.ascontiguousarray(nnz_per_row, np.int64)
+    pk = _Pack()
+    m = len(rp) - 1
+    mx = lib().orc_pack_mmmv(_p64(rp), m, int(rp[-1]), _p64(ps), _p64(nz), int(coef), C.byref(p), C.byref(pk))
     n = pk.total_cnt
     roff = [(pk.b_roff_maxc[i].key, pk.b_roff_maxc[i].val) for i in range(n + 1)]
     arr = [(pk.b_arr[i].key, pk.b_arr[i].val) for i in range(n)]

@@ -61,7 +61,15 @@ def lib():
     L.ams_hybacc_numc.argtypes = [P(CsrT), P(CsrT), P(CsrT), P(KeyVal), P(KeyVal), C.c_int64, P(C.c_int64),
                                   P(C.c_double), P(Bin)]
     L.ams_dropin_set_pack_params.argtypes = [C.c_int64, C.c_int64, C.c_double]
+    L.ams_hash_symb_omp_dnr.argtypes = [P(CsrT), P(CsrT), P(C.c_int64), P(Bin)]
+    L.ams_dropin_counters.argtypes = [P(C.c_int64), P(C.c_int64)]
     return L
+
+
+def counters(L):
+    up, sy = C.c_int64(), C.c_int64()
+    L.ams_dropin_counters(C.byref(up), C.byref(sy))
+    return up.value, sy.value
 
 
 class Held:
@@ -84,7 +92,9 @@ def test_whole_product(gpu_ctx, fn):
     a = gen.rmat_symmetric(11, 8, seed=3, values="uniform")
     ha = Held(a)
     c = CsrT()
+    up0, sy0 = counters(L)
     rt = getattr(L, fn)(C.byref(ha.s), C.byref(ha.s), C.byref(c), None)
+    assert counters(L) == (up0 + 1, sy0 + 1), "C = A*A uploads A once and plans once"
     assert rt.total_time > 0
     rp = as_np(c.row_ptrs, c.num_row + 1, np.int64)
     cols = as_np(c.cols, c.num_nnz, np.int64)
@@ -96,9 +106,11 @@ def test_whole_product(gpu_ctx, fn):
     np.testing.assert_allclose(rats, orats, rtol=1e-12, atol=1e-13)
 
 
-def test_phase_functions(gpu_ctx):
+@pytest.mark.parametrize("symb", ["ams_hash_symb_dnr", "ams_hash_symb_omp_dnr"])
+def test_phase_functions(gpu_ctx, symb):
     """The driver sequence of src/ams_hybaccspgemm.c:36-220 written against
-    the phase entry points."""
+    the phase entry points: A and B go to the GPU once each and the symbolic
+    phase runs once, its plan reused by the numeric phase."""
     L = lib()
     from oracle import oracle as orc
     a = random_csr(300, 200, 0.05, seed=7)
@@ -106,6 +118,7 @@ def test_phase_functions(gpu_ctx):
     ha, hb = Held(a), Held(b)
     m = a.num_row
     pack, binr = Pack(), Bin()
+    up0, sy0 = counters(L)
     L.init_ams_bin(C.byref(binr), C.byref(pack), C.byref(ha.s), None)
     L.flp_populate(C.byref(ha.s), C.byref(hb.s), binr.row_flp)
     flp = as_np(binr.row_flp, m, np.int64)
@@ -113,7 +126,7 @@ def test_phase_functions(gpu_ctx):
     assert L.get_intprod(C.byref(ha.s), C.byref(hb.s), binr.row_nz) == int(flp.sum())
     L.scan(binr.ps_row_flp, binr.row_flp, m + 1)
     np.testing.assert_array_equal(as_np(binr.ps_row_flp, m + 1, np.int64), np.concatenate([[0], np.cumsum(flp)]))
-    L.ams_hash_symb_dnr(C.byref(ha.s), C.byref(hb.s), None, C.byref(binr))
+    getattr(L, symb)(C.byref(ha.s), C.byref(hb.s), None, C.byref(binr))
     row_nz = as_np(binr.row_nz, m, np.int64)
     np.testing.assert_array_equal(row_nz, orc.hash_symb(a, b))
     crp = np.zeros(m + 1, np.int64)
@@ -136,8 +149,12 @@ def test_phase_functions(gpu_ctx):
     np.testing.assert_array_equal(crp, orp)
     np.testing.assert_array_equal(cols, ocols)
     np.testing.assert_allclose(rats, orats, rtol=1e-12, atol=1e-13)
+    assert counters(L) == (up0 + 2, sy0 + 1), "one upload of A and of B, one symbolic phase"
     L.free_ams_pack(C.byref(pack))
     L.free_ams_bin(C.byref(binr))
+    # after the sequence nothing is kept: the next call uploads again
+    L.flp_populate(C.byref(ha.s), C.byref(hb.s), None)
+    assert counters(L) == (up0 + 4, sy0 + 1)
 
 
 def test_numeric_needs_same_operands(gpu_ctx):
@@ -157,3 +174,27 @@ def test_numeric_needs_same_operands(gpu_ctx):
     torch.cuda.synchronize()
     assert gpu_ctx.error_flags() == 0
     assert c.num_nnz == nnz
+
+
+def test_numeric_reroutes_on_param_change(gpu_ctx):
+    """Numeric-routing parameters changed between the phases (here the
+    accumulator mode) only re-route the rows; parameters of the symbolic
+    geometry (the dense chunk width) require a new symbolic phase."""
+    import torch
+    from ams_hip import _lib as L
+    from ams_hip.device import Context, DeviceCsr
+    from helpers import assert_matches_oracle
+    a = gen.rmat_symmetric(10, 8, seed=4, values="uniform")
+    ctx = Context()
+    da = DeviceCsr.from_host(a)
+    rp, nnz = ctx.symbolic(da, da)
+    assert ctx.stats()["rows_num_esc"] > 0
+    ctx.set_params(mode=L.AMS_MODE_HASH)
+    c = ctx.numeric(da, da, rp, nnz)
+    torch.cuda.synchronize()
+    assert ctx.stats()["rows_num_esc"] == 0
+    assert_matches_oracle(c.to_host(), a, a, exact=False)
+    ctx.set_params(mode=L.AMS_MODE_HYBRID, dense_width=4096)
+    with pytest.raises(L.AmsError, match="symbolic parameters changed"):
+        ctx.numeric(da, da, rp, nnz)
+    ctx.close()

@@ -65,3 +65,54 @@ def test_stack_is_block_diagonal_product():
     assert abs(c - want).max() == 0
     assert list(ao) == [0, 5, 10, 15, 20] and list(bo) == [0, 7, 14, 21, 28]
     assert co[-1] == sum(9 + i for i in range(4))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_dropin_pack_mmmv_matches_oracle(seed):
+    """The drop-in's pack_ams_pack_mmmv (host bookkeeping inside
+    libams_hip.so, no GPU involved) builds the oracle's morsel table."""
+    import ctypes as C
+    from ams_hip import _lib
+    from oracle import oracle as orc
+
+    class KeyVal(C.Structure):
+        _fields_ = [("key", C.c_int64), ("val", C.c_double)]
+
+    class Pack(C.Structure):
+        _fields_ = [("b_roff_maxc", C.POINTER(KeyVal)), ("b_arr", C.POINTER(KeyVal)), ("total_cnt", C.c_int64),
+                    ("high_head_idx", C.c_int64), ("high_tail_idx", C.c_int64), ("low_head_idx", C.c_int64),
+                    ("low_tail_idx", C.c_int64)]
+
+    class CsrT(C.Structure):
+        _fields_ = [("num_row", C.c_int64), ("num_col", C.c_int64), ("num_nnz", C.c_int64),
+                    ("row_ptrs", C.POINTER(C.c_int64)), ("cols", C.POINTER(C.c_int64)),
+                    ("rats", C.POINTER(C.c_double))]
+
+    L = C.CDLL(_lib.LIB_PATH)
+    P64 = C.POINTER(C.c_int64)
+    L.pack_ams_pack_mmmv.argtypes = [C.POINTER(CsrT), C.POINTER(Pack), P64, P64, C.c_int64, C.c_void_p]
+    L.pack_ams_pack_mmmv.restype = C.c_int64
+    L.free_ams_pack.argtypes = [C.POINTER(Pack)]
+    L.ams_dropin_set_pack_params.argtypes = [C.c_int64, C.c_int64, C.c_double]
+    rng = np.random.default_rng(seed)
+    m = 500
+    nz = rng.integers(0, 60, m).astype(np.int64)
+    nz[rng.integers(0, m, 5)] = 400                       # rows longer than a morsel: empty morsels
+    rp = np.concatenate([[0], np.cumsum(nz)]).astype(np.int64)
+    coef = int(rng.integers(1, 4))
+    ps = coef * rp
+    csr = CsrT(m, 10, int(rp[-1]), rp.ctypes.data_as(P64), None, None)
+    L.ams_dropin_set_pack_params(256, 64, 0.85)
+    pk = Pack()
+    mx = L.pack_ams_pack_mmmv(C.byref(csr), C.byref(pk), ps.ctypes.data_as(P64), nz.ctypes.data_as(P64), coef,
+                              None)
+    p = orc.params(threads=1, ams_min_num=64, ams_size=256, ams_prt=0.85)
+    roff, arr, idx, omx = orc.pack_mmmv(rp, ps, nz, coef, p)
+    assert mx == omx and pk.total_cnt == idx["total_cnt"]
+    assert [(pk.b_roff_maxc[i].key, pk.b_roff_maxc[i].val) for i in range(pk.total_cnt + 1)] == roff
+    assert [(pk.b_arr[i].key, pk.b_arr[i].val) for i in range(pk.total_cnt)] == arr
+    assert (pk.low_head_idx, pk.high_head_idx, pk.high_tail_idx, pk.low_tail_idx) == (
+        idx["low_head_idx"], idx["high_head_idx"], idx["high_tail_idx"], idx["low_tail_idx"])
+    assert idx["low_head_idx"] > 0
+    L.free_ams_pack(C.byref(pk))
+    L.ams_dropin_set_pack_params(524288, 32768, 0.85)

@@ -125,6 +125,31 @@ def test_pack_tiny_input_guard():
     assert idx["total_cnt"] == 1 and [r[0] for r in roff] == [0, 2]
 
 
+def test_pack_mmmv_hand_computed():
+    # rows with 2, 0, 5, 1, 4 nonzeros: row_ptrs [0, 2, 2, 7, 8, 12];
+    # coef 2 and ps_row_coef = 2 * row_ptrs; T = MIN(4, 64/8) = 4
+    rp = np.array([0, 2, 2, 7, 8, 12], np.int64)
+    ps = 2 * rp                                           # [0, 4, 4, 14, 16, 24]
+    nz = np.array([2, 0, 5, 1, 4], np.int64)
+    p = orc.params(threads=1, ams_min_num=4, ams_size=64, ams_prt=0.5)
+    roff, arr, idx, mx = orc.pack_mmmv(rp, ps, nz, 2, p)
+    # total = 12 * 2 // 4 = 6; boundaries lower_bound(ps, 4k), k = 1..5:
+    # 4 -> 1, 8 -> 3, 12 -> 3, 16 -> 4, 20 -> 5; last = 5
+    assert idx["total_cnt"] == 6
+    assert [r[0] for r in roff] == [0, 1, 3, 3, 4, 5, 5]
+    # values: mean nonzeros per row of each morsel; [3, 3) and [5, 5) empty
+    vals = {0: 2 / 1, 1: 5 / 2, 2: 0.0, 3: 1 / 1, 4: 4 / 1, 5: 0.0}
+    assert [a[0] for a in arr] == [2, 5, 3, 0, 1, 4]
+    for k, v in arr:
+        assert v == pytest.approx(vals[k])
+    assert [r[1] for r in roff[:6]] == [2, 5, 0, 1, 4, 0]
+    assert mx == 5
+    assert idx["low_head_idx"] == 2
+    assert idx["high_head_idx"] == 5
+    assert idx["high_tail_idx"] == 4      # int((6 - 2) * 0.5 + 2)
+    assert idx["low_tail_idx"] == 3
+
+
 # ------------------------------------------------------------ radix sort
 
 def test_radix_sort_is_stable_lsd():
