@@ -95,6 +95,14 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// A non-empty B segment of a window of A nonzeros: product t of the window
+// (t in [off, off + len)) reads B position bias + t and is scaled by a.
+template <typename V>
+struct __align__(16) SegBias {
+  int64_t bias;
+  V a;
+};
+
 template <typename V>
 struct KeyVal {
   int32_t key;
@@ -490,30 +498,97 @@ __global__ __launch_bounds__(kScanT) void k_scan_apply(const int64_t* in, int64_
 
 // ------------------------------------------------------------- symbolic
 
-// LDS-hash symbolic, one row per block; table of S int32 keys.
+// LDS-hash symbolic, one row per block; table of S int32 keys (the row has
+// at most 2S/3 products).  Products are handed to threads in index order
+// (x = tid, tid + T, ...: 64 consecutive products of a B row per load) and
+// find their A nonzero without searching: the non-empty B segments of a
+// window of T A nonzeros are numbered, each sets the bit of its first
+// product in an S-bit map, and product x belongs to segment
+// (set bits at positions <= x) - 1, read from one (bits, bits before) word.
+#ifndef AMS_SYMB_HASH_U
+#define AMS_SYMB_HASH_U 4
+#endif
 template <int S, int T>
 __global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* list,
                                                  const int32_t* counts, int bin, int64_t* rownz,
                                                  int32_t* err) {
+  constexpr int U = AMS_SYMB_HASH_U;
+  constexpr int kWords = S / 32;
+  constexpr int kWpl = kWords >= kWave ? kWords / kWave : 1;
   __shared__ int32_t keys[S];
-  __shared__ ExpandLds<T> L;
+  __shared__ uint2 wpre[kWords];
+  __shared__ int64_t sbias[T];
   __shared__ int64_t s_w[T / kWave];
   constexpr int shift = 32 - __builtin_ctz(S);
   int32_t start, cnt;
   bin_range(counts, bin, start, cnt);
-  const SegB seg{A.col, B.rp};
   for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
     const int32_t row = list[start + idx];
+    const int64_t p0 = A.rp[row], p1 = A.rp[row + 1];
     for (int s = threadIdx.x; s < S; s += T) keys[s] = kEmpty;
-    __syncthreads();
     int64_t mine = 0;
-    block_expand<T, 2>(A.rp[row], A.rp[row + 1], L, seg,
-                       [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
-                       [&](int32_t key) {
-                         const int r = lds_insert_key(keys, shift, key);
-                         if (r > 0) ++mine;
-                         if (r < 0) atomicOr(err, 8);
-                       });
+    for (int64_t w0 = p0; w0 < p1; w0 += T) {
+      for (int i = threadIdx.x; i < kWords; i += T) wpre[i] = make_uint2(0u, 0u);
+      const int64_t p = w0 + threadIdx.x;
+      int64_t st = 0;
+      int32_t len = 0;
+      if (p < p1) {
+        const int32_t k = A.col[p];
+        st = B.rp[k];
+        len = (int32_t)(B.rp[k + 1] - st);
+      }
+      // the scan's barrier also orders the clearing above before the bits
+      int64_t tot64;
+      const int64_t ex = block_excl_scan_once<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
+      const int32_t o = (int32_t)(ex & 0xffffffffll), ci = (int32_t)(ex >> 32);
+      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
+      if (len > 0) {
+        sbias[ci] = st - o;
+        atomicOr(&wpre[o >> 5].x, 1u << (o & 31));
+      }
+      __syncthreads();
+      if (wave_id() == 0) {
+        uint32_t pc[kWpl];
+        int32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kWpl; ++q) {
+          const int wi = lane_id() * kWpl + q;
+          pc[q] = wi < kWords ? (uint32_t)__popc(wpre[wi].x) : 0u;
+          sum += (int32_t)pc[q];
+        }
+        int32_t run = wave_incl_scan(sum) - sum;
+#pragma unroll
+        for (int q = 0; q < kWpl; ++q) {
+          const int wi = lane_id() * kWpl + q;
+          if (wi < kWords) wpre[wi].y = (uint32_t)run;
+          run += (int32_t)pc[q];
+        }
+      }
+      __syncthreads();
+      for (int32_t x0 = 0; x0 < total; x0 += T * U) {
+        int32_t col[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t x = x0 + u * T + (int32_t)threadIdx.x;
+          ok[u] = x < total;
+          const int32_t xc = ok[u] ? x : 0;
+          const uint2 wp = wpre[xc >> 5];
+          const int32_t seg = (int32_t)wp.y + __popc(wp.x & ((2u << (xc & 31)) - 1u)) - 1;
+          const int64_t q = ok[u] ? sbias[seg < 0 ? 0 : seg] + xc : 0;
+          col[u] = ok[u] ? B.col[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (ok[u]) {
+            const int r = lds_insert_key(keys, shift, col[u]);
+            if (r > 0) ++mine;
+            if (r < 0) atomicOr(err, 8);
+          }
+        }
+      }
+      if (w0 + T < p1) __syncthreads();  // the next window rewrites the table
+    }
     const int64_t tot = block_sum<T, int64_t>(mine, s_w);
     if (threadIdx.x == 0) rownz[row] = tot;
   }
@@ -976,27 +1051,23 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
   }
 }
 
-// ESC of one row per block: expand the row's f <= E products into LDS
+// ESC of one row per block for columns too wide for 32-bit (column, rank)
+// keys, and the bucket sort: expand the row's f <= E products into LDS
 // (column and value, indexed by the product's rank in A-row order), sort the
 // ranks by (column, rank), then every run of equal columns is summed left to
 // right by the thread that owns its head.  The rank keeps equal columns in
 // A-row order, so each value is the same left-to-right sum of separately
 // rounded products as the reference's.  The sort is a bitonic sort of
-// (column << 32 | rank) words (esc_sort = 0) or a bucket sort of a rank
-// permutation (esc_sort = 1); both share one LDS region laid out as
+// (column << 32 | rank) words in registers (esc_sort 0 or 2) or a bucket
+// sort of a rank permutation (esc_sort 1); both share one LDS region:
 //   bucket:  cnt int32[E] | key uint32[E] | perm uint16[E]
-//   bitonic: sk uint64[E] (over cnt and key)   | key copy kept in regs
-// K32: the columns fit column << log2(E) | rank in 32 bits and the bitonic
-// sort is used; the sorted keys then go back into key[] and the LDS region
-// is 4E bytes instead of 10E (five 2048-product blocks per CU instead of
-// four).
+//   bitonic: sk uint64[E] (over cnt and key)
 // ESC expansion windows of at most 128 A nonzeros: for E = 2048 the block
-// then needs 26.2 KB of LDS instead of 27.7 KB, six blocks per CU instead
-// of five.
+// then needs 26.2 KB of LDS instead of 27.7 KB.
 template <int T>
 constexpr int kEscWin = T < 128 ? T : 128;
 
-template <int E, int T, typename V, bool K32>
+template <int E, int T, typename V>
 __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* list,
                                                 const int32_t* counts, int bin, const int64_t* row_flp,
                                                 const int64_t* rp_c, int32_t* col_c, V* val_c,
@@ -1004,11 +1075,11 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
   static_assert(E <= 65536, "ranks are uint16");
   constexpr int IPT = (E + T - 1) / T;
   constexpr int RB = __builtin_ctz(E);
-  __shared__ __align__(16) unsigned char region[K32 ? E * 4 : E * 10];
+  __shared__ __align__(16) unsigned char region[E * 10];
   int32_t* cnt = (int32_t*)region;
-  uint32_t* key = (uint32_t*)(region + (K32 ? 0 : 4 * E));
-  uint16_t* perm = (uint16_t*)(region + 8 * E);  // bucket sort only (not K32)
-  uint64_t* sk = (uint64_t*)region;             // 64-bit sorted keys (not K32)
+  uint32_t* key = (uint32_t*)(region + 4 * E);
+  uint16_t* perm = (uint16_t*)(region + 8 * E);  // bucket sort only
+  uint64_t* sk = (uint64_t*)region;             // 64-bit sorted keys
   __shared__ V vals[E];
   __shared__ ExpandLds<T, kEscWin<T>> L;  // rows of this bin have few A nonzeros
   __shared__ int64_t s_w[T / kWave];
@@ -1038,72 +1109,12 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
                          vals[r.rank] = r.a * r.val;
                        });
     const int nb = (int)next_pow2_i64(f);
-    if (K32 || esc_sort == 0) {
+    if (esc_sort != 1) {
       // (column, rank) keys sorted in registers: packed in 32 bits as
       // column << log2(E) | rank when the columns allow, else 64 bits
       static_assert(T * IPT == E, "the bin fills the threads' registers");
       const int i0 = threadIdx.x * IPT;
-      if constexpr (K32) {
-        uint32_t kr[IPT];
-#pragma unroll
-        for (int r = 0; r < IPT; ++r)
-          kr[r] = i0 + r < f ? ((key[i0 + r] << RB) | (uint32_t)(i0 + r)) : 0xffffffffu;
-        __syncthreads();
-        reg_bitonic_sort<T, IPT>(kr, key);
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) key[i0 + r] = kr[r];
-        __syncthreads();
-        // compress from the registers: thread t owns sorted elements
-        // [t*IPT, t*IPT + IPT); LDS is read only for the previous thread's
-        // last key and for a run that goes on into the next threads
-        const uint32_t pcol = (i0 > 0 && i0 <= f) ? (key[i0 - 1] >> RB) : 0u;
-        uint32_t hd = 0;
-        int64_t heads = 0;
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-          const bool head = i0 + r < f &&
-                            (r == 0 ? (i0 == 0 || (kr[0] >> RB) != pcol) : (kr[r] >> RB) != (kr[r - 1] >> RB));
-          hd |= (uint32_t)head << r;
-          heads += head;
-        }
-        int64_t tot;
-        int64_t rank = block_excl_scan<T, int64_t>(heads, s_w, tot);
-        if (threadIdx.x == 0 && tot != z) atomicOr(err, 2);
-        bool open = false;
-        uint32_t rc = 0;
-        V sum = V(0);
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-          if (i0 + r < f) {
-            const V x = vals[kr[r] & (uint32_t)(E - 1)];
-            if ((hd >> r) & 1u) {
-              if (open && rank < z) {
-                col_c[off + rank] = (int32_t)rc;
-                val_c[off + rank] = sum;
-              }
-              rank += open;
-              open = true;
-              rc = kr[r] >> RB;
-              sum = x;
-            } else if (open) {
-              sum += x;
-            }
-          }
-        }
-        if (open) {
-          for (int32_t g = i0 + IPT; g < f; ++g) {
-            const uint32_t kg = key[g];
-            if ((kg >> RB) != rc) break;
-            sum += vals[kg & (uint32_t)(E - 1)];
-          }
-          if (rank < z) {
-            col_c[off + rank] = (int32_t)rc;
-            val_c[off + rank] = sum;
-          }
-        }
-        __syncthreads();
-        continue;
-      } else {
+      {
         uint64_t kr[IPT];
 #pragma unroll
         for (int r = 0; r < IPT; ++r)
@@ -1114,7 +1125,7 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
         for (int r = 0; r < IPT; ++r) sk[i0 + r] = kr[r];
       }
       __syncthreads();
-    } else if constexpr (!K32) {
+    } else {
       uint32_t kmin = 0xffffffffu, kmax = 0u;
       for (int i = threadIdx.x; i < f; i += T) {
         kmin = min(kmin, key[i]);
@@ -1134,12 +1145,10 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
     }
     // sorted element e: column and rank
     auto col_at = [&](int e) -> uint32_t {
-      if constexpr (K32) return key[e] >> RB;
-      else return esc_sort == 0 ? (uint32_t)(sk[e] >> 32) : key[perm[e]];
+      return esc_sort != 1 ? (uint32_t)(sk[e] >> 32) : key[perm[e]];
     };
     auto rank_at = [&](int e) -> uint32_t {
-      if constexpr (K32) return key[e] & (uint32_t)(E - 1);
-      else return esc_sort == 0 ? (uint32_t)sk[e] : (uint32_t)perm[e];
+      return esc_sort != 1 ? (uint32_t)sk[e] : (uint32_t)perm[e];
     };
     const int32_t ipt = nb >= T ? nb / T : 1;
     const int32_t e0 = threadIdx.x * ipt;
@@ -1161,6 +1170,368 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
       ++rank;
     }
     __syncthreads();
+  }
+}
+
+// ESC of one row per block with (column << log2(E) | rank) keys in 32 bits.
+// Products are numbered by their rank t in A-row order (the order of the
+// reference's left-to-right sum) and held wave-contiguously: product t sits
+// in wave t / (64 IPT), register slot (t / 64) % IPT, lane t % 64, so each
+// load instruction reads 64 consecutive products of the row.  A product
+// finds its A nonzero without searching: the non-empty B segments of a
+// window of T A nonzeros are numbered, each sets the bit of its first
+// product in a bitmap, and product x belongs to segment
+//   (set bits at positions <= x) - 1,
+// one LDS read of the (bits, bits before) word pair.  The keys go straight
+// into the register bitonic sort; products a*b (separately rounded) wait in
+// LDS by rank.  After the sort each thread holds IPT consecutive keys; runs
+// of equal columns are summed left to right by the thread that owns their
+// head, and the sums go through LDS in rank order to coalesced stores.
+// LDS: vals V[E] | region of max(4E, 16T + E/4) bytes: during the expansion
+// the segment table (16 T bytes) and the bitmap word pairs (E / 4 bytes),
+// then the sort's cross-wave exchanges and the sorted keys, then the staged
+// columns.
+// Stable LSD radix sort of the first f of E keys by bits [lo, lo + bits),
+// 7 bits a pass, by the T threads of a block.  Element i lives in wave
+// i / (64 IPT), register slot (i / 64) % IPT, lane i % 64 ("wave-contiguous"),
+// so each wave's elements, chunk after chunk, are in index order.  A pass
+// ranks every element among the same-digit elements of its wave before it
+// (the lanes of a 64-element chunk with the same digit are found by seven
+// ballots; each wave keeps running counts per digit in LDS), one block scan
+// over (digit, wave) turns the counts into offsets, and the keys are
+// scattered through buf.  Equal digits keep their order, so equal columns
+// stay in rank order.  Only the f real keys take part; at the end buf[0, f)
+// holds the sorted keys.  cnt: 128 * (T / 64) ints.  Holds barriers.
+template <int T, int IPT>
+__device__ __forceinline__ void block_radix_sort(uint32_t (&kr)[IPT], int f, int lo, int bits, uint32_t* buf,
+                                                 int32_t* cnt, int64_t* s_w) {
+  constexpr int NW = T / kWave;
+  constexpr int kDigits = 128;
+  static_assert(kDigits * NW == 2 * T, "two counters a thread in the scan");
+  const int w = wave_id(), l = lane_id();
+  const unsigned long long below = (1ull << l) - 1ull;
+  for (int sh = lo; sh < lo + bits; sh += 7) {
+    cnt[2 * threadIdx.x] = 0;
+    cnt[2 * threadIdx.x + 1] = 0;
+    __syncthreads();
+    int32_t pos[IPT];
+#pragma unroll
+    for (int s = 0; s < IPT; ++s) {
+      const int i = (w * IPT + s) * kWave + l;
+      const bool act = i < f;
+      const uint32_t d = (kr[s] >> sh) & (kDigits - 1);
+      unsigned long long peers = __ballot(act);
+#pragma unroll
+      for (int b = 0; b < 7; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long m = __ballot(act && bit);
+        peers &= bit ? m : ~m;
+      }
+      const int32_t before = (int32_t)__popcll(peers & below);
+      int32_t* c = &cnt[d * NW + w];
+      const int32_t run = act ? *c : 0;
+      pos[s] = run + before;
+      if (act && before == 0) *c = run + (int32_t)__popcll(peers);
+      wave_lds_sync();  // the next chunk reads the counts this one wrote
+    }
+    __syncthreads();
+    // offsets of (digit, wave) in digit-major order
+    const int32_t c0 = cnt[2 * threadIdx.x], c1 = cnt[2 * threadIdx.x + 1];
+    int64_t tot;
+    const int32_t ex = (int32_t)block_excl_scan<T, int64_t>((int64_t)(c0 + c1), s_w, tot);
+    cnt[2 * threadIdx.x] = ex;
+    cnt[2 * threadIdx.x + 1] = ex + c0;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < IPT; ++s) {
+      const int i = (w * IPT + s) * kWave + l;
+      if (i < f) {
+        const uint32_t d = (kr[s] >> sh) & (kDigits - 1);
+        buf[cnt[d * NW + w] + pos[s]] = kr[s];
+      }
+    }
+    __syncthreads();
+    if (sh + 7 < lo + bits) {
+#pragma unroll
+      for (int s = 0; s < IPT; ++s) {
+        const int i = (w * IPT + s) * kWave + l;
+        kr[s] = i < f ? buf[i] : 0xffffffffu;
+      }
+    }
+  }
+}
+
+#ifndef AMS_ESC_GROUP
+#define AMS_ESC_GROUP 4
+#endif
+constexpr int kEscGroup = AMS_ESC_GROUP;  // products per thread in flight in the ESC expansion
+
+#ifndef AMS_ESC_WPE
+#define AMS_ESC_WPE 6
+#endif
+template <int E, int T, typename V, bool RADIX>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))) void k_numc_esc32(
+    Csr A, Csr B, const int32_t* list, const int32_t* counts, int bin, const int64_t* row_flp, const int64_t* rp_c,
+    int32_t* col_c, V* val_c, int64_t cap, int col_bits, int32_t* err) {
+  constexpr int IPT = E / T;
+  constexpr int RB = __builtin_ctz(E);
+  constexpr int kWords = E / 32;                               // bitmap words of a window
+  constexpr int kWpl = kWords >= kWave ? kWords / kWave : 1;   // words per lane of wave 0
+  constexpr int kRegion = 4 * E > 16 * T + 8 * kWords ? 4 * E : 16 * T + 8 * kWords;
+  static_assert(T * IPT == E && IPT <= 16, "the bin fills the threads' registers");
+  __shared__ __align__(16) unsigned char region[kRegion];
+  __shared__ V vals[E];
+  __shared__ int64_t s_w[T / kWave];
+  __shared__ int32_t s_cnt[RADIX ? 2 * T : 1];
+  uint32_t* key = (uint32_t*)region;
+  SegBias<V>* sb = (SegBias<V>*)region;
+  uint2* wpre = (uint2*)(region + 16 * T);
+  if (rp_c[A.m] > cap) return;
+  int32_t start, cnt_rows;
+  bin_range(counts, bin, start, cnt_rows);
+  const V* av = (const V*)A.val;
+  const V* bv = (const V*)B.val;
+  const int w = wave_id(), l = lane_id();
+  // Rows idx, idx + gridDim.x, ... of the bin.  A block works on one row at
+  // a time, so the next row's bounds, first A entries and B segment bounds
+  // are loaded while the current row is expanded, sorted and compressed.
+  // Row bounds travel as one dword per lane (lanes 0-3 C's row pointers,
+  // 4-7 A's, 8-9 the row's products), read out by readlane.
+  auto meta_load = [&](int32_t r) -> uint32_t {
+    uint32_t v = 0;
+    if (r >= 0 && l < 10) {
+      const uint32_t* src = l < 4 ? (const uint32_t*)(rp_c + r) + l
+                                  : (l < 8 ? (const uint32_t*)(A.rp + r) + (l - 4)
+                                           : (const uint32_t*)(row_flp + r) + (l - 8));
+      v = *src;
+    }
+    return v;
+  };
+  auto rd64 = [](uint32_t v, int lo) -> int64_t {
+    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, lo);
+    const uint32_t a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, lo + 1);
+    return (int64_t)(((uint64_t)a1 << 32) | a0);
+  };
+  int32_t idx = blockIdx.x;
+  if (idx >= cnt_rows) return;
+  uint32_t mv = meta_load(list[start + idx]);
+  // this thread's A entry of the row's first window and its B segment
+  int32_t pk = 0;
+  V pa = V(0);
+  int64_t pst = 0;
+  int32_t plen = 0;
+  {
+    const int64_t q0 = rd64(mv, 4) + threadIdx.x;
+    if (q0 < rd64(mv, 6)) {
+      pk = A.col[q0];
+      pa = av[q0];
+      pst = B.rp[pk];
+      plen = (int32_t)(B.rp[pk + 1] - pst);
+    }
+  }
+  for (; idx < cnt_rows; idx += gridDim.x) {
+    const int64_t off = rd64(mv, 0);
+    const int32_t z = (int32_t)(rd64(mv, 2) - off);
+    const int64_t p0 = rd64(mv, 4), p1 = rd64(mv, 6);
+    const int32_t f = (int32_t)rd64(mv, 8);
+    const int32_t nidx = idx + (int32_t)gridDim.x;
+    const int32_t nrow = nidx < cnt_rows ? list[start + nidx] : -1;  // used after the first scan
+    uint32_t nmv = 0;
+    uint32_t kr[IPT];
+#pragma unroll
+    for (int s = 0; s < IPT; ++s) kr[s] = 0xffffffffu;
+    int32_t base = 0;  // rank of the window's first product
+    for (int64_t w0 = p0; w0 < p1; w0 += T) {
+      for (int i = threadIdx.x; i < kWords; i += T) wpre[i] = make_uint2(0u, 0u);
+      int64_t st = pst;
+      int32_t len = plen;
+      V a = pa;
+      if (w0 != p0) {  // windows after the first (rows of more than T A nonzeros)
+        const int64_t p = w0 + threadIdx.x;
+        st = 0;
+        len = 0;
+        a = V(0);
+        if (p < p1) {
+          const int32_t k = A.col[p];
+          a = av[p];
+          st = B.rp[k];
+          len = (int32_t)(B.rp[k + 1] - st);
+        }
+      }
+      // offsets and compact indices of the non-empty segments (the scan's
+      // barrier also orders the bitmap clearing before the bits below)
+      int64_t tot64;
+      const int64_t ex = block_excl_scan_once<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
+      const int32_t o = (int32_t)(ex & 0xffffffffll), ci = (int32_t)(ex >> 32);
+      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
+      if (w0 == p0) nmv = meta_load(nrow);  // the next row's bounds
+      if (len > 0) {
+        sb[ci] = SegBias<V>{st - o, a};
+        atomicOr(&wpre[o >> 5].x, 1u << (o & 31));
+      }
+      __syncthreads();
+      if (w == 0) {  // set bits before each word
+        uint32_t pc[kWpl];
+        int32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kWpl; ++q) {
+          const int wi = l * kWpl + q;
+          pc[q] = wi < kWords ? (uint32_t)__popc(wpre[wi].x) : 0u;
+          sum += (int32_t)pc[q];
+        }
+        int32_t run = wave_incl_scan(sum) - sum;
+#pragma unroll
+        for (int q = 0; q < kWpl; ++q) {
+          const int wi = l * kWpl + q;
+          if (wi < kWords) wpre[wi].y = (uint32_t)run;
+          run += (int32_t)pc[q];
+        }
+      }
+      __syncthreads();
+      // this thread's products of the window, in groups of kEscGroup slots:
+      // a group's LDS reads and loads are all issued before any is used
+      static_for<0, (IPT + kEscGroup - 1) / kEscGroup>([&](auto gc) {
+        constexpr int g0 = decltype(gc)::value * kEscGroup;
+        constexpr int G = IPT - g0 < kEscGroup ? IPT - g0 : kEscGroup;
+        int32_t xq[G];
+        SegBias<V> sg[G];
+        int32_t cq[G];
+        V bq[G];
+        bool in[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int32_t t = (w * IPT + g0 + j) * kWave + l;
+          in[j] = t >= base && t < base + total;
+          xq[j] = in[j] ? t - base : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const uint2 wp = wpre[xq[j] >> 5];
+          const int32_t seg = (int32_t)wp.y + __popc(wp.x & ((2u << (xq[j] & 31)) - 1u)) - 1;
+          sg[j] = sb[seg < 0 ? 0 : seg];
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int64_t q = in[j] ? sg[j].bias + xq[j] : 0;
+          cq[j] = in[j] ? B.col[q] : 0;
+          bq[j] = in[j] ? bv[q] : V(0);
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          if (in[j]) {
+            const uint32_t t = (uint32_t)(base + xq[j]);
+            vals[t] = sg[j].a * bq[j];
+            kr[g0 + j] = ((uint32_t)cq[j] << RB) | t;
+          }
+        }
+      });
+      base += total;
+      if (w0 + T < p1) __syncthreads();  // the next window rewrites the table
+    }
+    // the next row's first A entries
+    pk = 0;
+    pa = V(0);
+    pst = 0;
+    plen = 0;
+    const int64_t q0 = rd64(nmv, 4) + threadIdx.x;
+    const bool pn = q0 < rd64(nmv, 6);
+    if (pn) {
+      pk = A.col[q0];
+      pa = av[q0];
+    }
+    __syncthreads();  // products in LDS; the region is free for the sort
+    const int i0 = threadIdx.x * IPT;
+    if constexpr (RADIX) {
+      block_radix_sort<T, IPT>(kr, f, RB, col_bits, key, s_cnt, s_w);
+    } else {
+      reg_bitonic_sort<T, IPT>(kr, key);
+    }
+    if (pn) {  // and their B segments, while the runs are compressed
+      pst = B.rp[pk];
+      plen = (int32_t)(B.rp[pk + 1] - pst);
+    }
+    if constexpr (RADIX) {  // this thread's IPT consecutive sorted keys
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) kr[r] = i0 + r < f ? key[i0 + r] : 0xffffffffu;
+    } else {
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) key[i0 + r] = kr[r];
+      __syncthreads();
+    }
+    // heads of runs of equal columns; the previous thread's last key from LDS
+    const uint32_t pcol = (i0 > 0 && i0 <= f) ? (key[i0 - 1] >> RB) : 0u;
+    uint32_t hd = 0;
+    int64_t heads = 0;
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      const bool head = i0 + r < f &&
+                        (r == 0 ? (i0 == 0 || (kr[0] >> RB) != pcol) : (kr[r] >> RB) != (kr[r - 1] >> RB));
+      hd |= (uint32_t)head << r;
+      heads += head;
+    }
+    int64_t tot;
+    const int64_t rank0 = block_excl_scan<T, int64_t>(heads, s_w, tot);
+    if (threadIdx.x == 0 && tot != z) atomicOr(err, 2);
+    // Runs are summed in registers.  The run that ends at element r (the last
+    // one before the next head, or the thread's last element when the run
+    // goes on into the next threads) keeps its column and sum in slot r, so
+    // every register index is a constant.
+    uint32_t ocol[IPT];
+    V osum[IPT];
+    uint32_t ends = 0;
+    bool open = false;
+    uint32_t rc = 0;
+    V sum = V(0);
+    static_for<0, IPT>([&](auto rr) {
+      constexpr int r = decltype(rr)::value;
+      ocol[r] = 0u;
+      osum[r] = V(0);
+      if (i0 + r < f) {
+        const V x = vals[kr[r] & (uint32_t)(E - 1)];
+        if ((hd >> r) & 1u) {
+          if constexpr (r > 0) {
+            if (open) {
+              ocol[r - 1] = rc;
+              osum[r - 1] = sum;
+              ends |= 1u << (r - 1);
+            }
+          }
+          open = true;
+          rc = kr[r] >> RB;
+          sum = x;
+        } else if (open) {
+          sum += x;
+        }
+      }
+    });
+    if (open) {
+      for (int32_t g = i0 + IPT; g < f; ++g) {
+        const uint32_t kg = key[g];
+        if ((kg >> RB) != rc) break;
+        sum += vals[kg & (uint32_t)(E - 1)];
+      }
+      ocol[IPT - 1] = rc;
+      osum[IPT - 1] = sum;
+      ends |= 1u << (IPT - 1);
+    }
+    __syncthreads();  // every read of the sorted keys and the products is done
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      if ((ends >> r) & 1u) {
+        const int64_t i = rank0 + __popc(ends & ((1u << r) - 1u));
+        if (i < z) {
+          key[i] = ocol[r];
+          vals[i] = osum[r];
+        }
+      }
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < z; i += T) {
+      col_c[off + i] = (int32_t)key[i];
+      val_c[off + i] = vals[i];
+    }
+    __syncthreads();  // before the next row reuses the LDS
+    mv = nmv;
   }
 }
 
@@ -1522,14 +1893,6 @@ __device__ __forceinline__ DenseTask task_get(uint32_t v) {
   d.cwidth = (int32_t)lane(9);
   return d;
 }
-
-// A non-empty segment of a dense window: product t of the window (t in
-// [off, off + len)) reads B position bias + t and is scaled by a.
-template <typename V>
-struct __align__(16) SegBias {
-  int64_t bias;
-  V a;
-};
 
 template <typename V>
 struct DenseItem {

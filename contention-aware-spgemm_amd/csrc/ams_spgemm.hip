@@ -755,11 +755,33 @@ template <int E, typename V, bool K32>
 static void launch_numc_esc_k(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
                               V* val_c, int64_t cap, int bin, int64_t cnt) {
   constexpr int T = hash_threads<E>();
+  if constexpr (K32) {
+    if (ctx->prm.esc_sort != 1) {
+      // register bitonic sort (esc_sort 0) or LSD radix sort by the column bits (2)
+      const int col_bits = B.n > 1 ? 64 - __builtin_clzll((unsigned long long)(B.n - 1)) : 1;
+      Ctrl* c = dctrl(ctx);
+      int occ;
+      if (ctx->prm.esc_sort == 2) {
+        occupancy(k_numc_esc32<E, T, V, true>, T, 0, &occ);
+        const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+        hipLaunchKernelGGL((k_numc_esc32<E, T, V, true>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                           (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
+                           (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, col_bits, &c->err);
+      } else {
+        occupancy(k_numc_esc32<E, T, V, false>, T, 0, &occ);
+        const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+        hipLaunchKernelGGL((k_numc_esc32<E, T, V, false>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                           (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
+                           (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, col_bits, &c->err);
+      }
+      return;
+    }
+  }
   int occ;
-  occupancy(k_numc_esc<E, T, V, K32>, T, 0, &occ);
+  occupancy(k_numc_esc<E, T, V>, T, 0, &occ);
   const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
   Ctrl* c = dctrl(ctx);
-  hipLaunchKernelGGL((k_numc_esc<E, T, V, K32>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+  hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
                      (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
                      (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
 }
@@ -770,7 +792,7 @@ static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const in
   const int64_t cnt = ctx->plan_counts ? ctx->cnt_num[bin] : A.m;
   if (ctx->plan_counts && cnt == 0) return;
   // (column, rank) fits 32 bits as column << log2(E) | rank below 2^(32 - log2(E)) columns
-  const bool key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(E))) && ctx->prm.esc_sort == 0;
+  const bool key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(E))) && ctx->prm.esc_sort != 1;
   if (key32) launch_numc_esc_k<E, V, true>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
   else launch_numc_esc_k<E, V, false>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
 }
@@ -1054,7 +1076,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
     return fail(ctx, AMS_ERR_ARG, "dense_width must be a multiple of 64 in [64, 65536]");
   if (p->bitmap_bits < p->dense_width) return fail(ctx, AMS_ERR_ARG, "bitmap_bits must be >= dense_width");
   if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
-  if (p->esc_sort < 0 || p->esc_sort > 1) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0 or 1");
+  if (p->esc_sort < 0 || p->esc_sort > 2) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0, 1 or 2");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   ctx->prm = *p;
   return AMS_OK;
@@ -1577,7 +1599,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   for (double t : thres) { ams_params_t p = best; p.acc_thres = t; consider(p); }
   for (double q : ratios) { ams_params_t p = best; p.bitmap_ratio = q; consider(p); }
   for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
-  for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
+  for (int e = 0; e < 3; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
   for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.sym_blocked = e; consider(p); }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

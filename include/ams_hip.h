@@ -62,7 +62,7 @@ typedef struct {
 	double  dense_density;  /* dense bin also when nnz(C row) >= density * n          */
 	int32_t mode;           /* AMS_MODE_*                                             */
 	int32_t wgs_per_cu;     /* persistent workgroups per CU for the queue kernels     */
-	int32_t esc_sort;       /* ESC row sort in LDS: 0 bitonic, 1 bucket               */
+	int32_t esc_sort;       /* ESC row sort: 0 register bitonic, 1 bucket, 2 LSD radix */
 	int32_t sym_blocked;    /* wide-row symbolic per (row, column chunk) on blocked B */
 } ams_params_t;
 

@@ -32,6 +32,7 @@ ROUTES = {
     "blocked_sym_dense": dict(sym_blocked=1, sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=128,
                               bitmap_bits=1024, dense_density=0.0),
     "esc_bucket": dict(mode=2, esc_sort=1),
+    "esc_radix": dict(mode=2, esc_sort=2),
 }
 
 
@@ -299,7 +300,7 @@ ESC_CASES = [("rmat_s8_ef4", lambda: gen.rmat_symmetric(8, 4, seed=101, values="
              ("rect_unsorted", None)]
 
 
-@pytest.mark.parametrize("esc_sort", [0, 1])
+@pytest.mark.parametrize("esc_sort", [0, 1, 2])
 @pytest.mark.parametrize("name,make", ESC_CASES, ids=[c[0] for c in ESC_CASES])
 def test_esc_random_values_bit_exact(gpu_ctx, name, make, esc_sort):
     """Random FP64 values through the ESC route: every row of at most esc_max
