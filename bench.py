@@ -44,6 +44,9 @@ WORKLOADS = {
     "parmat_g500_s19_ef32": dict(kind="parmat", scale=19, ef=32,
                                  text="G500 s19 ef32 with PaRMAT's construction (sorted, undirected, "
                                       "loop/dup-free, 8 columns of squares), mirrored, FP64 1.0, C = A*A^T"),
+    "wide_sparse": dict(kind="wide",
+                        text="uniform random A 65536 x 65536 (128 a row) times B 65536 x 2^24 (32 a row), "
+                             "FP64 1.0: rows of about 4096 products over 2^24 columns"),
     "join_agg": dict(kind="join",
                      text="join-aggregate incidence 850000 x 20000 (lognormal(2,1.5) degrees capped 5000, "
                           "Zipf(1.2) columns), FP64 1.0, C = A^T*A"),
@@ -66,6 +69,8 @@ def make_inputs(name):
         from ams_hip import parmat
         a = parmat.parmat_symmetric(w["scale"], w["ef"], gen.G500, seed=SEED + w["scale"])
         return a, a
+    if w["kind"] == "wide":
+        return gen.wide_sparse(seed=SEED)
     inc = gen.join_incidence(seed=SEED)
     return inc.transpose(), inc  # C = A^T A: csr_a = A^T, csr_b = A
 
@@ -76,20 +81,66 @@ def alg_bytes(phase, st, v, m):
       symbolic = 4 P + 12 nnz(A) + 16 (rows + 1)
       numeric  = (4 + v) P + (12 + v) nnz(A) + (4 + v) nnz(C) + 16 (rows + 1)
     restricted to the rows the phase processes."""
-    if phase in ("numc_dense", "numc_hash", "numc_esc"):
-        r = {"numc_hash": 0, "numc_esc": 1, "numc_dense": 2}[phase]
+    if phase in ("numc_dense", "numc_hash", "numc_esc", "numc_ghash"):
+        r = {"numc_hash": 0, "numc_esc": 1, "numc_dense": 2, "numc_ghash": 3}[phase]
         p, z, za = st["flops_route"][r], st["nnzc_route"][r], st["nnza_route"][r]
-        rows = [st["rows_num_hash"], st["rows_num_esc"], st["rows_num_dense"]][r]
+        rows = [st["rows_num_hash"], st["rows_num_esc"], st["rows_num_dense"], st["rows_num_ghash"]][r]
         return (4 + v) * p + (12 + v) * za + (4 + v) * z + 16 * (rows + 1)
-    if phase in ("symb_hash", "symb_bitmap"):
-        r = 0 if phase == "symb_hash" else 1
-        rows = st["rows_sym_hash"] if r == 0 else st["rows_sym_bitmap"]
+    if phase in ("symb_hash", "symb_bitmap", "symb_ghash"):
+        r = {"symb_hash": 0, "symb_bitmap": 1, "symb_ghash": 2}[phase]
+        rows = [st["rows_sym_hash"], st["rows_sym_bitmap"], st["rows_sym_ghash"]][r]
         return 4 * st["flops_sym"][r] + 12 * st["nnza_sym"][r] + 16 * (rows + 1)
     return None
 
 
 def whole_bytes(st, v, nnz_a, m):
     return (8 + v) * st["flops"] + (24 + v) * nnz_a + (4 + v) * st["nnz_c"] + 16 * (m + 1)
+
+
+KERNEL_OF_PHASE = {"numc_dense": "k_numc_dense", "symb_bitmap": "k_symb_bitmap", "numc_hash": "k_numc_hash",
+                   "symb_hash": "k_symb_hash", "numc_esc": "k_numc_esc", "symb_ghash": "k_symb_ghash",
+                   "numc_ghash": "k_numc_ghash"}
+
+
+def timed_steps(ctx, da, db, steps, warmup):
+    """warmup untimed one-shot products, then `steps` timed ones with the
+    library's phase events on; returns (ms per step, per-step ms, phase
+    timings {phase: (ms, launches)}, stats, last C)."""
+    import torch
+    c = None
+    for _ in range(warmup):
+        c = ctx.spgemm_pooled(da, db)
+        torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    torch.cuda.synchronize()
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        c = ctx.spgemm_pooled(da, db)  # one-shot: ends with the nnz readback
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    tm = ctx.timings()
+    ctx.enable_timing(False)
+    return ms, step_ms, tm, ctx.stats(), c
+
+
+def roofline(tm, st, v, m, workload):
+    """The dominant kernel's algorithmic bytes per launch over its
+    event-timed average launch, against the HBM peak."""
+    phases = {k: ms for k, (ms, n) in tm.items()}
+    dom = max(phases, key=phases.get)
+    ms_dom, n_dom = tm[dom]
+    bytes_dom = alg_bytes(dom, st, v, m)
+    if bytes_dom is None or n_dom == 0:
+        return None
+    ach = bytes_dom / (ms_dom / n_dom * 1e-3)
+    kernel = KERNEL_OF_PHASE[dom]
+    return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+            "traffic": read_traffic(kernel, workload), "kernel": kernel, "alg_bytes_per_launch": bytes_dom,
+            "avg_launch_ms": ms_dom / n_dom}
 
 
 def read_traffic(kernel, workload):
@@ -258,16 +309,63 @@ def bench_fwd_bwd(scale, ef, reps):
         ctx.masked_bwd(A, B, g)
     torch.cuda.synchronize()
     t_fb = (time.perf_counter() - t0) / reps
+    dA, dB = ctx.masked_bwd(A, B, g)
+    err = fp32_errors(ctx, A, B, C, g, dA, dB)
     ctx.close()
     return {"workload": f"G500 R-MAT s{scale} ef{ef}, FP32 N(0,1), C = A*B with B = A^T", "flops_P": P,
             "nnz_c": C.num_nnz, "fwd_ms": t_f * 1e3, "fwd_bwd_ms": t_fb * 1e3, "fwd_bwd_over_fwd": t_fb / t_f,
-            "fwd_gflops": 2.0 * P / t_f / 1e9}
+            "fwd_gflops": 2.0 * P / t_f / 1e9, "fp32_error_vs_fp64": err}
+
+
+def fp32_errors(ctx, A, B, C, g, dA, dB):
+    """Error of the FP32 forward and backward against the same products in
+    FP64 on the GPU (the FP64 path is held to 1e-12 of the oracle by the
+    tests): the largest |x32 - x64| / (|A||B|)_ij (the summation error bound
+    of the entry; FP32 rounding makes it at most k * 2^-24 for k terms) and
+    the largest |x32 - x64| / |x64| over entries above 1e-6 of the largest."""
+    import torch
+
+    from ams_hip.device import DeviceCsr
+
+    def cast(m, f=lambda v: v.double()):
+        return DeviceCsr(m.num_row, m.num_col, m.row_ptr, m.col, f(m.val))
+
+    def stats(x32, x64, bound):
+        worst_b, worst_r = 0.0, 0.0
+        big = 1e-6 * float(x64.abs().max())
+        step = 1 << 27
+        for s0 in range(0, x64.numel(), step):
+            e = (x32[s0:s0 + step].double() - x64[s0:s0 + step]).abs()
+            worst_b = max(worst_b, float((e / bound[s0:s0 + step].clamp_min(1e-300)).max()))
+            r = x64[s0:s0 + step].abs()
+            keep = r > big
+            if bool(keep.any()):
+                worst_r = max(worst_r, float((e[keep] / r[keep]).max()))
+        return {"max_err_over_abs_bound": worst_b, "max_rel_err": worst_r}
+
+    A64, B64 = cast(A), cast(B)
+    Aab, Bab = cast(A, lambda v: v.double().abs()), cast(B, lambda v: v.double().abs())
+    C64 = ctx.spgemm(A64, B64)
+    Cab = ctx.spgemm(Aab, Bab)
+    assert torch.equal(C64.col, C.col), "FP32 and FP64 products differ in structure"
+    out = {"C": stats(C.val, C64.val, Cab.val)}
+    del Cab
+    G64 = DeviceCsr(C64.num_row, C64.num_col, C64.row_ptr, C64.col, g.val.double())
+    Gab = DeviceCsr(C64.num_row, C64.num_col, C64.row_ptr, C64.col, g.val.double().abs())
+    dA64, dB64 = ctx.masked_bwd(A64, B64, G64)
+    bA, bB = ctx.masked_bwd(Aab, Bab, Gab)
+    out["dA"] = stats(dA, dA64, bA)
+    out["dB"] = stats(dB, dB64, bB)
+    del C64, G64, Gab
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_workload(name, reps):
-    """A few one-shot products of another workload (GFLOP/s = 2P/t)."""
-    import torch
-
+    """One-shot products of another workload (GFLOP/s = 2P/t), with the
+    dominant kernel's roofline and the whole pipeline's fraction of the HBM
+    roofline, and the all-ones checks (sum(C) = P, ascending columns)."""
     from ams_hip.device import Context, DeviceCsr
     t0 = time.perf_counter()
     a, b = make_inputs(name)
@@ -276,18 +374,18 @@ def bench_workload(name, reps):
     db = da if b is a else DeviceCsr.from_host(b)
     ctx = Context()
     _, P = ctx.row_flops(da, db)
-    c = ctx.spgemm_pooled(da, db)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        c = ctx.spgemm_pooled(da, db)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / reps
+    ms, step_ms, tm, st, c = timed_steps(ctx, da, db, reps, 1)
     ver = ctx.verify(c, 0)
+    wb = whole_bytes(st, 8, a.num_nnz, a.num_row)
     out = {"workload": WORKLOADS[name]["text"], "nnz_a": a.num_nnz, "flops_P": P, "nnz_c": ver["nnz"],
-           "ms_per_step": ms, "gflops": 2.0 * P / (ms * 1e-3) / 1e9, "sum_val_equals_P": ver["sum_val"] == float(P),
-           "unsorted_rows": ver["unsorted_rows"], "generate_s": gen_s}
-    del ctx
+           "ms_per_step": ms, "ms_per_step_median": float(np.median(step_ms)),
+           "gflops": 2.0 * P / (ms * 1e-3) / 1e9,
+           "roofline": roofline(tm, st, 8, a.num_row, name),
+           "pipeline_frac_hbm_roofline": wb / (ms * 1e-3) / HBM_PEAK,
+           "phase_ms_per_step": {k: v[0] / reps for k, v in tm.items() if v[0] > 0},
+           "sum_val_equals_P": ver["sum_val"] == float(P), "unsorted_rows": ver["unsorted_rows"],
+           "error_flags": ctx.error_flags(), "generate_s": gen_s}
+    ctx.close()
     return out
 
 
@@ -323,7 +421,8 @@ def main():
     ap.add_argument("--workload", default="g500_s19_ef32", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the batched and fwd+bwd sections")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched, fwd+bwd and other-workload sections")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuned rerun of the workload")
     ap.add_argument("--param", action="append", default=[], help="routing parameter key=value")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (HBM counter calibration) and exit")
@@ -363,48 +462,32 @@ def main():
         ctx.set_params(**kv)
     _, P = ctx.row_flops(da, db)
     log(f"P = {P}")
-    for i in range(args.warmup):
-        ctx.spgemm_pooled(da, db)
-        torch.cuda.synchronize()
-        log(f"warmup {i} done")
-    ctx.enable_timing(True)
-    torch.cuda.synchronize()
-    step_ms = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        c = ctx.spgemm_pooled(da, db)  # one-shot: ends with the nnz readback
-        torch.cuda.synchronize()
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    tm = ctx.timings()
-    ctx.enable_timing(False)
-    st = ctx.stats()
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    ms_step, step_ms, tm, st, c = timed_steps(ctx, da, db, args.steps, args.warmup)
     value = 2.0 * P / (ms_step * 1e-3) / 1e9
     log(f"{args.steps} steps, {ms_step:.3f} ms/step, {value:.1f} GFLOP/s; phases {tm}")
-
     v = 8
     phases = {k: ms for k, (ms, n) in tm.items()}
-    dom = max(phases, key=phases.get)
-    ms_dom, n_dom = tm[dom]
-    bytes_dom = alg_bytes(dom, st, v, a.num_row)
-    roof = None
-    if bytes_dom is not None and n_dom > 0:
-        per_launch_s = ms_dom / n_dom * 1e-3
-        ach = bytes_dom / per_launch_s
-        kernel = {"numc_dense": "k_numc_dense", "symb_bitmap": "k_symb_bitmap", "numc_hash": "k_numc_hash",
-                  "symb_hash": "k_symb_hash", "numc_esc": "k_numc_esc"}[dom]
-        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": read_traffic(kernel, args.workload),
-                "kernel": kernel, "alg_bytes_per_launch": bytes_dom, "avg_launch_ms": ms_dom / n_dom}
+    roof = roofline(tm, st, v, a.num_row, args.workload)
     wb = whole_bytes(st, v, a.num_nnz, a.num_row)
 
     params_used = ctx.params()
     ver = ctx.verify(c, 0)
     checks = {"nnz_c": ver["nnz"], "sum_col": ver["sum_col"], "sum_val": ver["sum_val"],
-              "unsorted_rows": ver["unsorted_rows"], "sum_val_equals_P": ver["sum_val"] == float(P)}
+              "unsorted_rows": ver["unsorted_rows"], "sum_val_equals_P": ver["sum_val"] == float(P),
+              "error_flags": ctx.error_flags()}
+
+    tuned = None
+    if not args.no_extras and not args.no_autotune:
+        # the autotuner's choice for this workload, outside the timed steps
+        t0 = time.perf_counter()
+        best, best_ms = ctx.autotune(da, db, reps=1)
+        tune_s = time.perf_counter() - t0
+        tms, _, _, _, _ = timed_steps(ctx, da, db, 3, 0)
+        changed = {k: best[k] for k in best if best[k] != params_used[k]}
+        tuned = {"changed_params": changed, "autotune_s": tune_s, "ms_per_step": tms,
+                 "gflops": 2.0 * P / (tms * 1e-3) / 1e9, "defaults_ms_per_step": ms_step}
+        log(f"autotuned: {tuned}")
+        ctx.set_params(**params_used)
 
     cpu = None
     if not args.no_cpu:
@@ -424,9 +507,10 @@ def main():
         log(f"fwd+bwd: {extras['fwd_bwd']}")
         extras["mtx_io"] = bench_mtx_io(16, 16)
         log(f"mtx io: {extras['mtx_io']}")
-        if args.workload != "parmat_g500_s19_ef32":
-            extras["parmat_g500_s19_ef32"] = bench_workload("parmat_g500_s19_ef32", 3)
-            log(f"parmat: {extras['parmat_g500_s19_ef32']}")
+        for name in ("er_s20_ef16", "join_agg", "parmat_g500_s19_ef32"):
+            if name != args.workload:
+                extras[name] = bench_workload(name, 5 if name != "parmat_g500_s19_ef32" else 3)
+                log(f"{name}: {extras[name]}")
 
     line = {
         "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
@@ -439,9 +523,11 @@ def main():
         "roofline": roof,
         "pipeline": {"alg_bytes_per_step": wb, "frac_hbm_roofline": wb / (ms_step * 1e-3) / HBM_PEAK,
                      "phase_ms_per_step": {k: ms / args.steps for k, ms in phases.items()},
-                     "routes": {k: st[k] for k in ("rows_sym_hash", "rows_sym_bitmap", "rows_num_hash",
-                                                   "rows_num_esc", "rows_num_dense", "dense_chunks")}},
+                     "routes": {k: st[k] for k in ("rows_sym_hash", "rows_sym_bitmap", "rows_sym_ghash",
+                                                   "rows_num_hash", "rows_num_esc", "rows_num_dense",
+                                                   "rows_num_ghash", "dense_chunks")}},
         "checks": checks,
+        "autotuned": tuned,
         "cpu_baseline": cpu,
         **extras,
     }

@@ -27,7 +27,7 @@ class AmsParams(C.Structure):
                 ("acc_thres", C.c_double), ("dense_width", C.c_int32), ("bitmap_bits", C.c_int32),
                 ("bitmap_ratio", C.c_double), ("dense_density", C.c_double), ("mode", C.c_int32),
                 ("wgs_per_cu", C.c_int32), ("esc_sort", C.c_int32),
-                ("sym_blocked", C.c_int32)]
+                ("sym_blocked", C.c_int32), ("ghash_ratio", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -37,8 +37,9 @@ class AmsStats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("rows_total", "rows_empty", "rows_sym_hash", "rows_sym_bitmap",
                                          "rows_num_hash", "rows_num_esc", "rows_num_dense",
                                          "dense_chunks", "flops", "nnz_c")] + [
-        ("flops_route", C.c_int64 * 3), ("nnzc_route", C.c_int64 * 3), ("nnza_route", C.c_int64 * 3),
-        ("flops_sym", C.c_int64 * 2), ("nnza_sym", C.c_int64 * 2)]
+        ("flops_route", C.c_int64 * 4), ("nnzc_route", C.c_int64 * 4), ("nnza_route", C.c_int64 * 4),
+        ("flops_sym", C.c_int64 * 3), ("nnza_sym", C.c_int64 * 3), ("rows_sym_ghash", C.c_int64),
+        ("rows_num_ghash", C.c_int64)]
 
     def as_dict(self):
         out = {}
@@ -49,7 +50,7 @@ class AmsStats(C.Structure):
 
 
 PHASES = ["flops", "classify", "symb_hash", "symb_bitmap", "scan", "bblk", "numc_dense", "numc_hash",
-          "numc_esc"]
+          "numc_esc", "symb_ghash", "numc_ghash"]
 
 
 # name -> (restype, argtypes); every symbol declared in include/ams_hip.h.

@@ -148,3 +148,22 @@ def batch_problems(count=512, seed=0, scales=(8, 14), efs=(2, 4, 8, 16, 32)):
     ef = rng.choice(np.asarray(efs), count)
     return [rmat_symmetric(int(s), int(e), G500, seed=seed * 100003 + i, values="uniform")
             for i, (s, e) in enumerate(zip(sc, ef))]
+
+
+def wide_sparse(m=65536, k=65536, n=1 << 24, deg_a=128, deg_b=32, seed=0):
+    """A (m x k, about deg_a nonzeros a row) and B (k x n, about deg_b a row)
+    with uniform random columns, values 1.0: rows of C = A B have about
+    deg_a * deg_b products spread over n columns, so they are long for an
+    LDS table yet very sparse next to their column range."""
+    rng = np.random.default_rng(seed)
+
+    def rand_csr(rows, cols, deg):
+        r = np.repeat(np.arange(rows, dtype=np.int64), deg)
+        c = rng.integers(0, cols, rows * deg, dtype=np.int64)
+        keys = np.unique(r * cols + c)
+        r, c = keys // cols, keys % cols
+        rp = np.zeros(rows + 1, np.int64)
+        np.cumsum(np.bincount(r, minlength=rows), out=rp[1:])
+        return Csr(rows, cols, rp, c.astype(np.int32), np.ones(len(c), np.float64))
+
+    return rand_csr(m, k, deg_a), rand_csr(k, n, deg_b)

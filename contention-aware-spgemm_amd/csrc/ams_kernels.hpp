@@ -21,14 +21,14 @@
 namespace ams {
 
 // Symbolic routes.
-enum : uint8_t { SB_NONE = 0xff, SB_BITMAP = 8 };  // 0..7: LDS hash, table 64 << b
+enum : uint8_t { SB_NONE = 0xff, SB_BITMAP = 8, SB_GHASH = 9 };  // 0..7: LDS hash, table 64 << b
 constexpr int kSymHashBins = 8;
-constexpr int kSymBins = 9;
+constexpr int kSymBins = 10;
 // Numeric routes.
-enum : uint8_t { NB_NONE = 0xff, NB_DENSE = 16 };  // 0..7 hash (64 << b), 8..14 ESC (64 << (b-8))
+enum : uint8_t { NB_NONE = 0xff, NB_DENSE = 16, NB_GHASH = 17 };  // 0..7 hash (64 << b), 8..14 ESC (64 << (b-8))
 constexpr int kNumHashBins = 8;
 constexpr int kNumEscBins = 7;
-constexpr int kNumBins = 17;
+constexpr int kNumBins = 18;
 constexpr int kMaxHashSlots = 64 << (kNumHashBins - 1);  // 8192
 constexpr int kMaxEsc = 64 << (kNumEscBins - 1);         // 4096
 
@@ -86,6 +86,7 @@ struct SymParams {
   int64_t sym_hash_max;
   double bitmap_ratio;
   int64_t bitmap_bits;  // usable LDS bitmap width (0 disables the bitmap route)
+  double ghash_ratio;   // global hash for rows with sym_hash_max < f < ratio * words (0: off)
 };
 
 struct NumParams {
@@ -93,7 +94,21 @@ struct NumParams {
   int64_t esc_max;
   double acc_thres;
   double dense_density;
-  int32_t mode;  // 0 hybrid, 1 hash, 2 esc
+  int32_t mode;    // 0 hybrid, 1 hash, 2 esc
+  int32_t vbytes;  // bytes of a value (global-hash table sizes)
 };
+
+// Slots of a global hash table for n keys: a power of two >= 1.5 n.
+__device__ __forceinline__ int64_t ghash_slots(int64_t n) {
+  const int64_t want = n + ((n + 1) >> 1);
+  return want <= 2 ? 2 : (int64_t)1 << (64 - __clzll((unsigned long long)(want - 1)));
+}
+
+// Bytes of a row's numeric global-hash region: keys and values of the
+// table, then the z columns and values the sort passes through.
+__device__ __forceinline__ int64_t ghash_num_bytes(int64_t z, int vbytes) {
+  const int64_t s = ghash_slots(z);
+  return ((s * (4 + vbytes) + z * (4 + vbytes) + 8) + 15) & ~(int64_t)15;
+}
 
 }  // namespace ams

@@ -348,11 +348,17 @@ __device__ __forceinline__ int64_t row_width(const Probs& pr, int64_t row) {
 }
 
 // Symbolic route of every row.  Rows without products get nnz 0 here.
+// Rows too long for an LDS table whose products are few next to their
+// column range (f < ghash_ratio * words) take the global-hash route: a
+// bitmap pass would walk the whole range for them.  Its table bytes are
+// summed into *ghash_bytes.
 __global__ __launch_bounds__(256) void k_classify_sym(int64_t m, const int64_t* row_flp, Probs pr,
                                                       SymParams sp, uint8_t* sbin, int64_t* rownz,
-                                                      int32_t* counts) {
+                                                      int32_t* counts, unsigned long long* ghash_bytes) {
   __shared__ int32_t s_cnt[kMaxBins];
+  __shared__ int64_t s_w[256 / kWave];
   block_count_begin(s_cnt);
+  int64_t gb = 0;
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = i0 + threadIdx.x;
     const bool valid = i < m;
@@ -363,7 +369,10 @@ __global__ __launch_bounds__(256) void k_classify_sym(int64_t m, const int64_t* 
         rownz[i] = 0;
       } else {
         const int64_t words = (row_width(pr, i) + 31) >> 5;
-        if (f > sp.sym_hash_max || (double)f >= sp.bitmap_ratio * (double)words) {
+        if (f > sp.sym_hash_max && (double)f < sp.ghash_ratio * (double)words) {
+          b = SB_GHASH;
+          gb += 4 * ghash_slots(f);
+        } else if (f > sp.sym_hash_max || (double)f >= sp.bitmap_ratio * (double)words) {
           b = SB_BITMAP;
         } else {
           const int64_t s = next_pow2_i64(f + ((f + 1) >> 1));
@@ -374,16 +383,22 @@ __global__ __launch_bounds__(256) void k_classify_sym(int64_t m, const int64_t* 
     }
     (void)wave_count_lds(s_cnt, b, b != SB_NONE);
   }
-  block_count_end(s_cnt, counts, SB_BITMAP + 1);
+  block_count_end(s_cnt, counts, kSymBins);
+  const int64_t t = block_sum<256, int64_t>(gb, s_w);
+  if (threadIdx.x == 0 && t) atomicAdd(ghash_bytes, (unsigned long long)t);
 }
 
-// Numeric route of every row (needs C's row pointers).
+// Numeric route of every row (needs C's row pointers).  Rows of the
+// global-hash symbolic route that neither ESC nor an LDS table takes stay
+// in global hash tables; their region bytes are summed into *ghash_bytes.
 __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* row_flp,
                                                       const int64_t* rp_c, const uint8_t* sbin,
                                                       Probs pr, NumParams np, uint8_t* nbin,
-                                                      int32_t* counts) {
+                                                      int32_t* counts, unsigned long long* ghash_bytes) {
   __shared__ int32_t s_cnt[kMaxBins];
+  __shared__ int64_t s_w[256 / kWave];
   block_count_begin(s_cnt);
+  int64_t gb = 0;
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < m; i0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = i0 + threadIdx.x;
     const bool valid = i < m;
@@ -403,6 +418,9 @@ __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* 
         } else if (z <= np.num_hash_max) {
           const int64_t s = next_pow2_i64(z + ((z + 1) >> 1));
           b = s <= 64 ? 0 : ilog2_pow2(s) - 6;
+        } else if (sbin[i] == SB_GHASH) {
+          b = NB_GHASH;
+          gb += ghash_num_bytes(z, np.vbytes);
         } else {
           b = NB_DENSE;
         }
@@ -412,6 +430,8 @@ __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* 
     (void)wave_count_lds(s_cnt, b, b != NB_NONE);
   }
   block_count_end(s_cnt, counts, kNumBins);
+  const int64_t t = block_sum<256, int64_t>(gb, s_w);
+  if (threadIdx.x == 0 && t) atomicAdd(ghash_bytes, (unsigned long long)t);
 }
 
 // Rows grouped by bin: bin b occupies list[sum(counts[0..b)) ...).
@@ -589,6 +609,58 @@ __global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* li
       }
       if (w0 + T < p1) __syncthreads();  // the next window rewrites the table
     }
+    const int64_t tot = block_sum<T, int64_t>(mine, s_w);
+    if (threadIdx.x == 0) rownz[row] = tot;
+  }
+}
+
+// Global-hash symbolic: one block per row, distinct columns counted in a
+// table of ghash_slots(f) int32 keys in global memory (the row's products
+// are few next to its column range, so an LDS bitmap pass would mostly walk
+// empty words, and the row is too long for an LDS table).  The restatement
+// of the reference's per-row table sized from the row's products
+// (src/ams/ams_utils.c:1600-1752) without an LDS bound.  Tables come from
+// gtab by an atomic cursor (in keys); the host sized gtab from the classify
+// kernel's sum.  Linear probing, insert by compare-and-swap.
+__device__ __forceinline__ int ghash_insert_key(int32_t* tab, int lg, int32_t key) {
+  const uint32_t mask = (1u << lg) - 1u;  // lg <= 31
+  uint32_t h = lg == 0 ? 0u : ((uint32_t)key * kHashMul) >> (32 - lg);
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const int32_t old = atomicCAS(&tab[h], kEmpty, key);
+    if (old == kEmpty) return 1;
+    if (old == key) return 0;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void k_symb_ghash(Csr A, Csr B, const int64_t* row_flp, const int32_t* list,
+                                                  const int32_t* counts, int bin, int64_t* rownz, int32_t* gtab,
+                                                  unsigned long long* cursor, int32_t* err) {
+  __shared__ ExpandLds<T> L;
+  __shared__ int64_t s_w[T / kWave];
+  __shared__ unsigned long long s_base;
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const SegB seg{A.col, B.rp};
+  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+    const int32_t row = list[start + idx];
+    const int64_t S = ghash_slots(row_flp[row]);
+    const int lg = ilog2_pow2(S);
+    if (threadIdx.x == 0) s_base = atomicAdd(cursor, (unsigned long long)S);
+    __syncthreads();
+    int32_t* tab = gtab + s_base;
+    for (int64_t i = threadIdx.x; i < S; i += T) tab[i] = kEmpty;
+    __syncthreads();
+    int64_t mine = 0;
+    block_expand<T, 4>(A.rp[row], A.rp[row + 1], L, seg,
+                       [&](int, int64_t, int64_t q, int64_t) { return B.col[q]; },
+                       [&](int32_t key) {
+                         const int r = ghash_insert_key(tab, lg, key);
+                         if (r > 0) ++mine;
+                         if (r < 0) atomicOr(err, 0x40);
+                       });
     const int64_t tot = block_sum<T, int64_t>(mine, s_w);
     if (threadIdx.x == 0) rownz[row] = tot;
   }
@@ -1074,7 +1146,6 @@ __global__ __launch_bounds__(T) void k_numc_esc(Csr A, Csr B, const int32_t* lis
                                                 int64_t cap, int esc_sort, int32_t* err) {
   static_assert(E <= 65536, "ranks are uint16");
   constexpr int IPT = (E + T - 1) / T;
-  constexpr int RB = __builtin_ctz(E);
   __shared__ __align__(16) unsigned char region[E * 10];
   int32_t* cnt = (int32_t*)region;
   uint32_t* key = (uint32_t*)(region + 4 * E);
@@ -1532,6 +1603,187 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
     }
     __syncthreads();  // before the next row reuses the LDS
     mv = nmv;
+  }
+}
+
+// Stable LSD radix sort of n (key, value) pairs in global memory by key
+// bits [0, bits), 7 bits a pass, by one block: wave w owns the index range
+// [w n / NW, (w + 1) n / NW) and walks it 64 pairs at a time, so a pass
+// counts each (digit, wave) over its range (same-digit lanes of a step found
+// by seven ballots), one block scan turns the counts into offsets, and a
+// second walk scatters the pairs into the other buffer.  Returns true when
+// the result ended in (k1, v1).  cnt: 128 * NW ints of LDS.  Holds barriers.
+template <int T, typename V>
+__device__ bool block_radix_sort_global(int32_t* k0, V* v0, int32_t* k1, V* v1, int64_t n, int bits, int32_t* cnt,
+                                        int64_t* s_w) {
+  constexpr int NW = T / kWave;
+  constexpr int kDigits = 128;
+  const int w = wave_id(), l = lane_id();
+  const unsigned long long below = (1ull << l) - 1ull;
+  const int64_t lo = n * w / NW, hi = n * (w + 1) / NW;
+  bool flip = false;
+  auto peers_of = [&](bool act, uint32_t d) {
+    unsigned long long peers = __ballot(act);
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long mb = __ballot(act && bit);
+      peers &= bit ? mb : ~mb;
+    }
+    return peers;
+  };
+  for (int sh = 0; sh < bits; sh += 7) {
+    const int32_t* sk = flip ? k1 : k0;
+    const V* sv = flip ? v1 : v0;
+    int32_t* dk = flip ? k0 : k1;
+    V* dv = flip ? v0 : v1;
+    for (int i = threadIdx.x; i < kDigits * NW; i += T) cnt[i] = 0;
+    __syncthreads();
+    for (int64_t i0 = lo; i0 < hi; i0 += kWave) {
+      const int64_t i = i0 + l;
+      const bool act = i < hi;
+      const uint32_t d = act ? ((uint32_t)sk[i] >> sh) & (kDigits - 1) : 0u;
+      const unsigned long long peers = peers_of(act, d);
+      if (act && (peers & below) == 0) cnt[d * NW + w] += (int32_t)__popcll(peers);
+      wave_lds_sync();
+    }
+    __syncthreads();
+    constexpr int PER = (kDigits * NW + T - 1) / T;
+    int32_t part[PER];
+    int32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x * PER + q;
+      part[q] = e < kDigits * NW ? cnt[e] : 0;
+      sum += part[q];
+    }
+    int64_t tot;
+    int32_t run = (int32_t)block_excl_scan<T, int64_t>((int64_t)sum, s_w, tot);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x * PER + q;
+      if (e < kDigits * NW) cnt[e] = run;
+      run += part[q];
+    }
+    __syncthreads();
+    for (int64_t i0 = lo; i0 < hi; i0 += kWave) {
+      const int64_t i = i0 + l;
+      const bool act = i < hi;
+      int32_t k = 0;
+      V v = V(0);
+      if (act) {
+        k = sk[i];
+        v = sv[i];
+      }
+      const uint32_t d = ((uint32_t)k >> sh) & (kDigits - 1);
+      const unsigned long long peers = peers_of(act, d);
+      const int32_t before = (int32_t)__popcll(peers & below);
+      int32_t* c = &cnt[d * NW + w];
+      const int32_t base = act ? *c : 0;
+      if (act) {
+        dk[base + before] = k;
+        dv[base + before] = v;
+      }
+      if (act && before == 0) *c = base + (int32_t)__popcll(peers);
+      wave_lds_sync();
+    }
+    __syncthreads();  // the pass's pairs are written before the next one reads them
+    flip = !flip;
+  }
+  return flip;
+}
+
+// Global-hash numeric: one block per row of the global-hash route.  Products
+// accumulate into a table of ghash_slots(z) keys and values in global memory
+// (compare-and-swap of the key, atomic add of the separately rounded
+// product), the occupied slots are compacted into C's row, and the row is
+// radix-sorted by column through the region's scratch pairs.  Region bytes
+// come from gbuf by an atomic cursor (ghash_num_bytes per row).
+template <int T, typename V>
+__global__ __launch_bounds__(T) void k_numc_ghash(Csr A, Csr B, const int32_t* list, const int32_t* counts, int bin,
+                                                  const int64_t* rp_c, int32_t* col_c, V* val_c, int64_t cap,
+                                                  unsigned char* gbuf, unsigned long long* cursor, int col_bits,
+                                                  int32_t* err) {
+  __shared__ ExpandLds<T> L;
+  __shared__ V aval[T];
+  __shared__ int64_t s_w[T / kWave];
+  __shared__ unsigned long long s_base;
+  __shared__ int32_t s_n;
+  __shared__ int32_t s_cnt[128 * (T / kWave)];
+  if (rp_c[A.m] > cap) return;
+  int32_t start, cnt;
+  bin_range(counts, bin, start, cnt);
+  const V* av = (const V*)A.val;
+  const V* bv = (const V*)B.val;
+  for (int32_t idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+    const int32_t row = list[start + idx];
+    const int64_t off = rp_c[row];
+    const int64_t z = rp_c[row + 1] - off;
+    const int64_t S = ghash_slots(z);
+    const int lg = ilog2_pow2(S);
+    const uint32_t mask = (uint32_t)(S - 1);
+    if (threadIdx.x == 0) {
+      s_base = atomicAdd(cursor, (unsigned long long)ghash_num_bytes(z, (int)sizeof(V)));
+      s_n = 0;
+    }
+    __syncthreads();
+    int32_t* keys = (int32_t*)(gbuf + s_base);
+    V* vals = (V*)(gbuf + s_base + (((uint64_t)S * 4 + 7) & ~7ull));
+    int32_t* tk = (int32_t*)(vals + S);
+    V* tv = (V*)(gbuf + s_base + (((((uint64_t)S * 4 + 7) & ~7ull) + (uint64_t)S * sizeof(V) + (uint64_t)z * 4 + 7) & ~7ull));
+    for (int64_t i = threadIdx.x; i < S; i += T) {
+      keys[i] = kEmpty;
+      vals[i] = V(0);
+    }
+    __syncthreads();
+    block_expand<T, 4>(A.rp[row], A.rp[row + 1], L,
+                       [&](int64_t p, int64_t& st, int32_t& len) {
+                         const int32_t k = A.col[p];
+                         st = B.rp[k];
+                         len = (int32_t)(B.rp[k + 1] - st);
+                         aval[threadIdx.x] = av[p];
+                       },
+                       [&](int j, int64_t, int64_t q, int64_t) { return Gather<V>{B.col[q], j, bv[q]}; },
+                       [&](Gather<V> g) {
+                         const V x = aval[g.j] * g.val;
+                         uint32_t h = lg == 0 ? 0u : ((uint32_t)g.key * kHashMul) >> (32 - lg);
+                         for (uint32_t probe = 0; probe <= mask; ++probe) {
+                           const int32_t old = atomicCAS(&keys[h], kEmpty, g.key);
+                           if (old == kEmpty || old == g.key) {
+                             atomicAdd(&vals[h], x);
+                             return;
+                           }
+                           h = (h + 1) & mask;
+                         }
+                         atomicOr(err, 0x80);
+                       });
+    // compact the occupied slots into C's row (any order; sorted below)
+    for (int64_t s0 = 0; s0 < S; s0 += T) {
+      const int64_t sl = s0 + threadIdx.x;
+      const int32_t key = sl < S ? keys[sl] : kEmpty;
+      const bool occ = key != kEmpty;
+      const unsigned long long mk = __ballot(occ);
+      int32_t base = 0;
+      if (lane_id() == 0 && mk) base = atomicAdd(&s_n, (int32_t)__popcll(mk));
+      base = __shfl(base, 0, kWave);
+      const int32_t r = base + (int32_t)__popcll(mk & ((1ull << lane_id()) - 1ull));
+      if (occ && r < z) {
+        col_c[off + r] = key;
+        val_c[off + r] = vals[sl];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n != z) atomicOr(err, 0x20);
+    if (s_n == z && z > 1) {
+      const bool in_t = block_radix_sort_global<T, V>(col_c + off, val_c + off, tk, tv, z, col_bits, s_cnt, s_w);
+      if (in_t) {
+        for (int64_t i = threadIdx.x; i < z; i += T) {
+          col_c[off + i] = tk[i];
+          val_c[off + i] = tv[i];
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -2298,11 +2550,11 @@ __global__ void k_copy_i32(const int32_t* src, int32_t* dst) {
 }
 
 // Work per route, for reports and per-kernel rooflines:
-//   sums[r]     products of numeric route r (0 hash, 1 ESC, 2 dense)
-//   sums[3 + r] nnz(C) of route r, sums[6 + r] nnz(A) of route r
-//   sums[9 + s] products of symbolic route s (0 hash, 1 bitmap)
-//   sums[11 + s] nnz(A) of symbolic route s
-constexpr int kRouteSums = 13;
+//   sums[r]      products of numeric route r (0 hash, 1 ESC, 2 dense, 3 global hash)
+//   sums[4 + r]  nnz(C) of route r, sums[8 + r] nnz(A) of route r
+//   sums[12 + s] products of symbolic route s (0 hash, 1 bitmap, 2 global hash)
+//   sums[15 + s] nnz(A) of symbolic route s
+constexpr int kRouteSums = 18;
 __global__ __launch_bounds__(256) void k_route_sums(int64_t m, const int64_t* row_flp, const int64_t* rp_c,
                                                     const int64_t* rp_a, const uint8_t* sbin,
                                                     const uint8_t* nbin, unsigned long long* sums) {
@@ -2312,19 +2564,23 @@ __global__ __launch_bounds__(256) void k_route_sums(int64_t m, const int64_t* ro
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t f = row_flp[i], z = rp_c[i + 1] - rp_c[i], d = rp_a[i + 1] - rp_a[i];
     const int nb = nbin[i], sb = sbin[i];
-    const int r = nb == NB_NONE ? -1 : (nb == NB_DENSE ? 2 : (nb >= kNumHashBins ? 1 : 0));
+    const int r = nb == NB_NONE ? -1
+                                : (nb == NB_GHASH ? 3 : (nb == NB_DENSE ? 2 : (nb >= kNumHashBins ? 1 : 0)));
     if (r >= 0) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
+      for (int k = 0; k < 4; ++k) {
         acc[k] += (r == k) ? f : 0;
-        acc[3 + k] += (r == k) ? z : 0;
-        acc[6 + k] += (r == k) ? d : 0;
+        acc[4 + k] += (r == k) ? z : 0;
+        acc[8 + k] += (r == k) ? d : 0;
       }
     }
     if (sb != SB_NONE) {
-      const int q = sb == SB_BITMAP ? 1 : 0;
-      acc[9 + q] += f;
-      acc[11 + q] += d;
+      const int q = sb == SB_GHASH ? 2 : (sb == SB_BITMAP ? 1 : 0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        acc[12 + k] += (q == k) ? f : 0;
+        acc[15 + k] += (q == k) ? d : 0;
+      }
     }
   }
 #pragma unroll

@@ -45,11 +45,14 @@ struct Ctrl {
   int64_t ntask;
   int32_t nlong;  // long B rows listed for the wave-per-row blocked copy
   int32_t pad1;
-  unsigned long long sums[16];
+  unsigned long long sums[24];     // route sums [0, 18), checksums of C [20, 23)
   int32_t nheavy;   // heavy rows of the bitmap bin split into pieces
   int32_t npieces;
   unsigned long long sym_ticket;  // next row of the bitmap bin to count
-  int64_t pad2[3];
+  unsigned long long ghash_sym_bytes;  // global-hash tables of the symbolic route
+  unsigned long long ghash_num_bytes;  // global-hash regions of the numeric route
+  unsigned long long ghash_cursor;     // next free table slot / byte
+  int64_t pad2;
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 
@@ -125,6 +128,8 @@ struct ams_ctx {
   Buf row_flp, sbin, nbin, lsym, lnum, ctrl, coff, bptr, bcol, bval, bperm, tsum;
   Buf pool_rp, pool_col, pool_val, probs, prow, pbrow, scratch_a, dflag, dtask, ccnt, lrows;
   Buf hflag, hrow, hpiece, gbm;  // heavy bitmap rows: flags, slots, pieces, global bitmaps
+  Buf gtab, gnum;                // global-hash tables: symbolic keys, numeric regions
+  int64_t ghash_num_bytes = 0;   // of the last symbolic plan
   size_t gbm_zero = 0;           // bytes of gbm known to be zero
   bool bblk_ready = false;  // blocked B of the current plan already built
   Ctrl* h_ctrl = nullptr;       // pinned mirror of the control block
@@ -510,13 +515,13 @@ static int route_numeric(ams_ctx_t* ctx, const Csr& A, const Probs& pr, const in
   const int64_t m = A.m;
   Ctrl* c = dctrl(ctx);
   const NumParams np{ctx->prm.num_hash_max, ctx->prm.esc_max, ctx->prm.acc_thres, ctx->prm.dense_density,
-               ctx->prm.mode};
+                     ctx->prm.mode, ctx->plan_dtype == AMS_F64 ? 8 : 4};
   if (m > 0) {
     Phase ph(ctx, AMS_PH_CLASSIFY, sync);
     const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
     hipLaunchKernelGGL(k_classify_num, dim3(gc), dim3(256), 0, ctx->stream, m,
                        (const int64_t*)ctx->row_flp.p, (const int64_t*)rp_c, (const uint8_t*)ctx->sbin.p, pr,
-                       np, (uint8_t*)ctx->nbin.p, c->cnt_num);
+                       np, (uint8_t*)ctx->nbin.p, c->cnt_num, &c->ghash_num_bytes);
     hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->nbin.p,
                        kNumBins, (const int32_t*)c->cnt_num, c->cur_num, (int32_t*)ctx->lnum.p);
     if (sync)
@@ -542,17 +547,21 @@ static int route_numeric(ams_ctx_t* ctx, const Csr& A, const Probs& pr, const in
     for (int i = 0; i < kNumHashBins; ++i) s.rows_num_hash += ctx->cnt_num[i];
     for (int i = 0; i < kNumEscBins; ++i) s.rows_num_esc += ctx->cnt_num[kNumHashBins + i];
     s.rows_num_dense = ctx->cnt_num[NB_DENSE];
+    s.rows_sym_ghash = ctx->cnt_sym[SB_GHASH];
+    s.rows_num_ghash = ctx->cnt_num[NB_GHASH];
+    s.rows_empty -= s.rows_sym_ghash;
     s.dense_chunks = R;
     s.nnz_c = ctx->h_pin[0];
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       s.flops_route[k] = (int64_t)ctx->h_ctrl->sums[k];
-      s.nnzc_route[k] = (int64_t)ctx->h_ctrl->sums[3 + k];
-      s.nnza_route[k] = (int64_t)ctx->h_ctrl->sums[6 + k];
+      s.nnzc_route[k] = (int64_t)ctx->h_ctrl->sums[4 + k];
+      s.nnza_route[k] = (int64_t)ctx->h_ctrl->sums[8 + k];
     }
-    for (int k = 0; k < 2; ++k) {
-      s.flops_sym[k] = (int64_t)ctx->h_ctrl->sums[9 + k];
-      s.nnza_sym[k] = (int64_t)ctx->h_ctrl->sums[11 + k];
+    for (int k = 0; k < 3; ++k) {
+      s.flops_sym[k] = (int64_t)ctx->h_ctrl->sums[12 + k];
+      s.nnza_sym[k] = (int64_t)ctx->h_ctrl->sums[15 + k];
     }
+    ctx->ghash_num_bytes = (int64_t)ctx->h_ctrl->ghash_num_bytes;
   }
   return AMS_OK;
 }
@@ -562,7 +571,8 @@ static int route_numeric(ams_ctx_t* ctx, const Csr& A, const Probs& pr, const in
 static int reroute_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const int64_t* rp_c) {
   Ctrl* c = dctrl(ctx);
   CHK(fill_i32(ctx, c->cnt_num, 2 * 32, 0));  // cnt_num and cur_num
-  CHK(fill_i64(ctx, (int64_t*)c->sums, 13, 0));
+  CHK(fill_i64(ctx, (int64_t*)c->sums, 18, 0));
+  CHK(fill_i64(ctx, (int64_t*)&c->ghash_num_bytes, 1, 0));
   CHK(route_numeric(ctx, to_csr(a), ctx->plan_pr, rp_c, ctx->plan_R, true, nullptr));
   ctx->plan_prm = ctx->prm;
   return AMS_OK;
@@ -599,7 +609,9 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   ctx->plan_brp = B.rp;
   ctx->plan_bcol = B.col;
 
-  SymParams sp{ctx->prm.sym_hash_max, ctx->prm.bitmap_ratio, Wb};
+  // the global-hash route needs the sizes of its tables on the host: only
+  // in synchronised calls
+  SymParams sp{ctx->prm.sym_hash_max, ctx->prm.bitmap_ratio, Wb, sync ? ctx->prm.ghash_ratio : 0.0};
   if (m > 0) {
     {
       Phase ph(ctx, AMS_PH_FLOPS, sync);
@@ -608,7 +620,8 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     Phase ph(ctx, AMS_PH_CLASSIFY, sync);
     const int gc = (int)std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cu * 8);
     hipLaunchKernelGGL(k_classify_sym, dim3(gc), dim3(256), 0, ctx->stream, m,
-                       (const int64_t*)ctx->row_flp.p, pr, sp, (uint8_t*)ctx->sbin.p, rp_c, c->cnt_sym);
+                       (const int64_t*)ctx->row_flp.p, pr, sp, (uint8_t*)ctx->sbin.p, rp_c, c->cnt_sym,
+                       &c->ghash_sym_bytes);
     hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->sbin.p,
                        kSymBins, (const int32_t*)c->cnt_sym, c->cur_sym, (int32_t*)ctx->lsym.p);
     HIPCHK(hipGetLastError());
@@ -636,6 +649,18 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     launch_symb_hash<2048>(ctx, A, B, rownz, 5);
     launch_symb_hash<4096>(ctx, A, B, rownz, 6);
     launch_symb_hash<8192>(ctx, A, B, rownz, 7);
+    HIPCHK(hipGetLastError());
+  }
+  if (m > 0 && ctx->plan_counts && ctx->cnt_sym[SB_GHASH] > 0) {
+    Phase ph(ctx, AMS_PH_SYMB_GHASH, sync);
+    CHK(grow(ctx, ctx->gtab, (size_t)ctx->h_ctrl->ghash_sym_bytes));
+    constexpr int TG = 256;
+    int occ;
+    occupancy(k_symb_ghash<TG>, TG, 0, &occ);
+    const int g = grid_for(ctx, occ, ctx->cnt_sym[SB_GHASH], true);
+    hipLaunchKernelGGL(k_symb_ghash<TG>, dim3(g), dim3(TG), 0, ctx->stream, A, B, (const int64_t*)ctx->row_flp.p,
+                       (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_GHASH, rp_c,
+                       (int32_t*)ctx->gtab.p, &c->ghash_cursor, &c->err);
     HIPCHK(hipGetLastError());
   }
   ctx->bblk_ready = false;
@@ -867,6 +892,20 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
     launch_numc_hash<4096, V>(ctx, A, B, rp_c, col_c, val_c, cap, 6);
     launch_numc_hash<8192, V>(ctx, A, B, rp_c, col_c, val_c, cap, 7);
   }
+  if (A.m > 0 && ctx->plan_counts && ctx->cnt_num[NB_GHASH] > 0) {
+    Phase ph(ctx, AMS_PH_NUMC_GHASH, on);
+    CHK(grow(ctx, ctx->gnum, (size_t)ctx->ghash_num_bytes));
+    CHK(fill_i64(ctx, (int64_t*)&c->ghash_cursor, 1, 0));
+    constexpr int TG = 256;
+    int occ;
+    occupancy(k_numc_ghash<TG, V>, TG, 0, &occ);
+    const int g = grid_for(ctx, occ, ctx->cnt_num[NB_GHASH], true);
+    const int col_bits = B.n > 1 ? 64 - __builtin_clzll((unsigned long long)(B.n - 1)) : 1;
+    hipLaunchKernelGGL((k_numc_ghash<TG, V>), dim3(g), dim3(TG), 0, ctx->stream, A, B, (const int32_t*)ctx->lnum.p,
+                       (const int32_t*)c->cnt_num, (int)NB_GHASH, rp_c, col_c, val_c, cap,
+                       (unsigned char*)ctx->gnum.p, &c->ghash_cursor, col_bits, &c->err);
+    HIPCHK(hipGetLastError());
+  }
   if (A.m > 0) {
     Phase ph(ctx, AMS_PH_NUMC_ESC, on);
     launch_numc_esc<64, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 0);
@@ -1000,6 +1039,11 @@ void ams_default_params(ams_params_t* p) {
   p->wgs_per_cu = 0;
   p->esc_sort = 0;
   p->sym_blocked = 0;
+  // rows of fewer products than a quarter of their range's words: G500 s19
+  // ef32 unchanged (390.4 against 391.4 ms), 2^24-column rows of about 4096
+  // products 765 -> 26 ms; 1.0 would send G500 rows that dense chunks do
+  // 2.8 times faster to the global hash
+  p->ghash_ratio = 0.25;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -1043,7 +1087,8 @@ int ams_ctx_destroy(ams_ctx_t* ctx) {
   Buf* bufs[] = {&ctx->row_flp, &ctx->sbin, &ctx->nbin, &ctx->lsym, &ctx->lnum, &ctx->ctrl, &ctx->coff,
                  &ctx->bptr, &ctx->bcol, &ctx->bval, &ctx->bperm, &ctx->tsum, &ctx->pool_rp, &ctx->pool_col,
                  &ctx->pool_val, &ctx->probs, &ctx->scratch_a, &ctx->dflag, &ctx->dtask, &ctx->ccnt,
-                 &ctx->prow, &ctx->pbrow, &ctx->lrows, &ctx->hflag, &ctx->hrow, &ctx->hpiece, &ctx->gbm};
+                 &ctx->prow, &ctx->pbrow, &ctx->lrows, &ctx->hflag, &ctx->hrow, &ctx->hpiece, &ctx->gbm,
+                 &ctx->gtab, &ctx->gnum};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : ctx->recs) {
@@ -1078,6 +1123,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
   if (p->esc_sort < 0 || p->esc_sort > 2) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0, 1 or 2");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
+  if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
   ctx->prm = *p;
   return AMS_OK;
 }
@@ -1200,7 +1246,7 @@ int ams_verify_result(ams_ctx_t* ctx, const ams_dcsr_t* c, int dtype, int64_t* n
   CHK(check_csr(ctx, c, "C"));
   CHK(grow(ctx, ctx->ctrl, sizeof(Ctrl)));
   Ctrl* d = dctrl(ctx);
-  unsigned long long* out = &d->sums[13];  // [13] col sum, [14] unsorted rows, [15] value sum
+  unsigned long long* out = &d->sums[20];  // [20] col sum, [21] unsorted rows, [22] value sum
   CHK(fill_i64(ctx, (int64_t*)out, 3, 0));
   const Csr C = to_csr(c);
   if (C.m > 0) {
@@ -1404,14 +1450,15 @@ int ams_batched_run(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, in
 // the duration of one ams_spgemm_fixed call and restored afterwards.
 struct FixedSwap {
   ams_ctx_t* ctx;
-  static constexpr int kN = 27;
+  static constexpr int kN = 29;
   Buf* bufs[kN];
   Buf saved[kN];
   explicit FixedSwap(ams_ctx_t* c) : ctx(c) {
     Buf* all[kN] = {&c->row_flp, &c->sbin, &c->nbin, &c->lsym, &c->lnum, &c->ctrl, &c->coff,
                     &c->bptr, &c->bcol, &c->bval, &c->bperm, &c->tsum, &c->pool_rp, &c->pool_col,
                     &c->pool_val, &c->probs, &c->scratch_a, &c->dflag, &c->dtask, &c->ccnt,
-                    &c->prow, &c->pbrow, &c->lrows, &c->hflag, &c->hrow, &c->hpiece, &c->gbm};
+                    &c->prow, &c->pbrow, &c->lrows, &c->hflag, &c->hrow, &c->hpiece, &c->gbm,
+                    &c->gtab, &c->gnum};
     for (int i = 0; i < kN; ++i) {
       bufs[i] = all[i];
       saved[i] = *all[i];
@@ -1587,6 +1634,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   const double thres[] = {0.0, 1.25, 2.0, 4.0};
   const double ratios[] = {0.25, 1.0, 4.0};
   const double dens[] = {0.02, 0.05, 0.2, 1.0};
+  const double ghash[] = {0.0, 0.25, 1.0};
   auto consider = [&](ams_params_t p) {
     double ms;
     if (timed(p, &ms) == AMS_OK && ms < best_ms) {
@@ -1601,6 +1649,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
   for (int e = 0; e < 3; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
   for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.sym_blocked = e; consider(p); }
+  for (double g : ghash) { ams_params_t p = best; p.ghash_ratio = g; consider(p); }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->err.clear();

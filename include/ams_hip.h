@@ -64,6 +64,9 @@ typedef struct {
 	int32_t wgs_per_cu;     /* persistent workgroups per CU for the queue kernels     */
 	int32_t esc_sort;       /* ESC row sort: 0 register bitonic, 1 bucket, 2 LSD radix */
 	int32_t sym_blocked;    /* wide-row symbolic per (row, column chunk) on blocked B */
+	double  ghash_ratio;    /* rows with flops > sym_hash_max and flops < ratio * words(n)
+	                           count and accumulate in global-memory hash tables
+	                           (synchronised calls; 0 = off)                      */
 } ams_params_t;
 
 /* Counters of the last symbolic / numeric call (rows per bin, tasks). */
@@ -78,19 +81,21 @@ typedef struct {
 	int64_t dense_chunks;      /* column chunks of the dense bin                  */
 	int64_t flops;             /* P = sum of row flops                            */
 	int64_t nnz_c;
-	/* per numeric route (hash, esc, dense): products, nnz(C), nnz(A) */
-	int64_t flops_route[3];
-	int64_t nnzc_route[3];
-	int64_t nnza_route[3];
-	/* per symbolic route (hash, bitmap): products, nnz(A) */
-	int64_t flops_sym[2];
-	int64_t nnza_sym[2];
+	/* per numeric route (hash, esc, dense, global hash): products, nnz(C), nnz(A) */
+	int64_t flops_route[4];
+	int64_t nnzc_route[4];
+	int64_t nnza_route[4];
+	/* per symbolic route (hash, bitmap, global hash): products, nnz(A) */
+	int64_t flops_sym[3];
+	int64_t nnza_sym[3];
+	int64_t rows_sym_ghash;
+	int64_t rows_num_ghash;
 } ams_stats_t;
 
 /* Phases timed with HIP events on the context stream when timing is on. */
 enum { AMS_PH_FLOPS = 0, AMS_PH_CLASSIFY = 1, AMS_PH_SYMB_HASH = 2, AMS_PH_SYMB_BITMAP = 3,
        AMS_PH_SCAN = 4, AMS_PH_BBLK = 5, AMS_PH_NUMC_DENSE = 6, AMS_PH_NUMC_HASH = 7,
-       AMS_PH_NUMC_ESC = 8, AMS_PH_COUNT = 9 };
+       AMS_PH_NUMC_ESC = 8, AMS_PH_SYMB_GHASH = 9, AMS_PH_NUMC_GHASH = 10, AMS_PH_COUNT = 11 };
 
 typedef struct ams_ctx ams_ctx_t;
 
@@ -206,7 +211,9 @@ int ams_spgemm_fixed(ams_ctx_t *ctx, const ams_dcsr_t *a, const ams_dcsr_t *b, i
  * stream drains (so it also serves after a graph replay or a batched run,
  * which do not synchronise): 0x1 LDS-hash row count differs from its
  * symbolic count, 0x2 ESC row count differs, 0x4 dense chunk count
- * differs, 0x8 symbolic hash table full, 0x10 numeric hash table full.
+ * differs, 0x8 symbolic hash table full, 0x10 numeric hash table full,
+ * 0x20 global-hash row count differs, 0x40 / 0x80 symbolic / numeric
+ * global hash table full.
  * Any non-zero value means C is not to be trusted.  No counterpart in the
  * reference (its kernels cannot detect these). */
 int ams_get_error_flags(ams_ctx_t *ctx, int32_t *flags);

@@ -33,6 +33,8 @@ ROUTES = {
                               bitmap_bits=1024, dense_density=0.0),
     "esc_bucket": dict(mode=2, esc_sort=1),
     "esc_radix": dict(mode=2, esc_sort=2),
+    "ghash": dict(sym_hash_max=64, num_hash_max=64, ghash_ratio=1e9, mode=1),
+    "ghash_hybrid": dict(sym_hash_max=64, num_hash_max=64, ghash_ratio=1e9),
 }
 
 
@@ -71,7 +73,7 @@ def test_rmat_square(gpu_ctx, route, name, a, exact):
     assert_matches_oracle(c, a, a, exact)
 
 
-@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only", "blocked_sym_dense", "esc_bucket"])
+@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only", "blocked_sym_dense", "esc_bucket", "ghash"])
 def test_rectangular_unsorted(gpu_ctx, route):
     a = random_csr(70, 300, 0.05, seed=1)
     b = random_csr(300, 2000, 0.02, seed=2, sort=False)
@@ -338,3 +340,63 @@ def test_esc_random_values_bit_exact(gpu_ctx, name, make, esc_sort):
     assert np.all(np.abs(c.rats - vals) <= 1e-12 * bound + 1e-300)
     print(f"[parity] {name} esc_sort={esc_sort}: {int(esc_rows.sum())} of {len(vals)} entries from ESC rows "
           f"bit-exact", flush=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_global_hash_route(gpu_ctx, dtype):
+    """Rows too long for an LDS table and sparse next to their column range
+    go to global-memory hash tables (symbolic and numeric) when ghash_ratio
+    allows; the rows come back sorted and equal to the oracle's."""
+    import torch
+    from ams_hip.device import DeviceCsr
+    from ams_hip._lib import AmsParams, lib
+    a = gen.rmat_symmetric(12, 16, seed=61, values="uniform")
+    if dtype == np.float32:  # the oracle sees the values the GPU multiplies
+        a = Csr(a.num_row, a.num_col, a.row_ptrs, a.cols, a.rats.astype(np.float32).astype(np.float64))
+    p = AmsParams()
+    lib().ams_default_params(p)
+    # rows of more than 64 products and fewer than 40 * words(4096) = 5120
+    # take the global-hash symbolic; without ESC (acc_thres 0) and with LDS
+    # tables up to 64 entries their numeric phase does too
+    gpu_ctx.set_params(**{**p.as_dict(), "ghash_ratio": 40.0, "sym_hash_max": 64, "num_hash_max": 64,
+                          "acc_thres": 0.0})
+    da = DeviceCsr.from_host(a, val_dtype=dtype)
+    c = gpu_ctx.spgemm(da, da)
+    torch.cuda.synchronize()
+    st = gpu_ctx.stats()
+    print(f"[parity] ghash {np.dtype(dtype).name}: rows sym {st['rows_sym_ghash']} num {st['rows_num_ghash']}",
+          flush=True)
+    assert st["rows_sym_ghash"] > 0 and st["rows_num_ghash"] > 0
+    h = c.to_host()
+    assert h.sorted_rows()
+    if dtype == np.float64:
+        assert_matches_oracle(h, a, a, exact=False)
+    else:
+        # FP32 sums of at most ~16000 terms: |error| <= 16000 * 2^-24 * (|A||B|)_ij
+        assert_matches_oracle(h, a, a, exact=False, rel=1e-3)
+
+
+def test_global_hash_wide_sparse(gpu_ctx):
+    """Very wide sparse rows (4,000,000 columns, a few thousand products a
+    row): all-ones values, bit-exact against the oracle with the global
+    hash on and off."""
+    import torch
+    from ams_hip.device import DeviceCsr
+    from ams_hip._lib import AmsParams, lib
+    rng = np.random.default_rng(5)
+    m, k, n = 300, 2000, 4_000_000
+    a = random_csr(m, k, 0.05, seed=51, values="ones")
+    deg = rng.integers(20, 60, k)
+    rp = np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+    cols = np.concatenate([np.sort(rng.choice(n, d, replace=False)) for d in deg]).astype(np.int32)
+    b = Csr(k, n, rp, cols, np.ones(len(cols)))
+    p = AmsParams()
+    lib().ams_default_params(p)
+    for ratio in (0.0, 1.0):
+        gpu_ctx.set_params(**{**p.as_dict(), "ghash_ratio": ratio})
+        c = gpu_ctx.spgemm(DeviceCsr.from_host(a), DeviceCsr.from_host(b))
+        torch.cuda.synchronize()
+        st = gpu_ctx.stats()
+        print(f"[parity] wide sparse ghash_ratio={ratio}: ghash rows {st['rows_num_ghash']}", flush=True)
+        assert (st["rows_num_ghash"] > 0) == (ratio > 0)
+        assert_matches_oracle(c.to_host(), a, b, exact=True)
