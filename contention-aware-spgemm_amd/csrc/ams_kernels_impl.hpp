@@ -916,17 +916,21 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
   // (the bin list's order, and so the rows' sizes, vary from run to run); one
   // at a time when the bin has few rows per block
   __shared__ int32_t s_next;
+  // thread 0's batch of tickets, kept in LDS: in registers it would stay
+  // live (and spill) across every row's product loop
+  __shared__ unsigned long long s_qtk, s_qend;
   const unsigned long long per_block = (unsigned long long)cnt / ((unsigned long long)gridDim.x * 16ull);
   const unsigned long long grab = per_block < 1 ? 1 : (per_block > kSymGrab ? kSymGrab : per_block);
-  unsigned long long q_tk = 0, q_end = 0;  // thread 0's batch of tickets
+  if (threadIdx.x == 0) s_qtk = s_qend = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      if (q_tk == q_end) {
+      unsigned long long q_tk = s_qtk;
+      if (q_tk == s_qend) {
         q_tk = atomicAdd(row_ticket, grab);
-        q_end = q_tk + grab;
+        s_qend = q_tk + grab;
       }
       s_next = q_tk < (unsigned long long)cnt ? (int32_t)q_tk : cnt;
-      ++q_tk;
+      s_qtk = q_tk + 1;
     }
     __syncthreads();
     const int32_t h = s_next;
