@@ -70,6 +70,20 @@ struct SegB {
   }
 };
 
+// The same with 32-bit byte offsets from the arrays' bases (A with fewer
+// than 2^30 nonzeros, B with fewer than 2^28 rows): the loads take a scalar
+// base and one vector offset register, and no 64-bit address stays live.
+struct SegB32 {
+  const int32_t* acol;
+  const int64_t* brp;
+  __device__ __forceinline__ void operator()(int64_t p, int64_t& st, int32_t& len) const {
+    const int32_t k = *(const int32_t*)((const char*)acol + ((uint32_t)p << 2));
+    const int64_t* b = (const int64_t*)((const char*)brp + ((uint32_t)k << 3));
+    st = b[0];
+    len = (int32_t)(b[1] - st);
+  }
+};
+
 // Products in flight per lane in the expansion (loads issued before the
 // LDS updates they feed), per kernel.
 #ifndef AMS_SYMB_U
@@ -823,10 +837,12 @@ __global__ __launch_bounds__(T) void k_heavy_finish(const Probs pr, const int32_
 // nonzeros [p0, p1) in the LDS bitmap bm.  Hub columns recur: the bit is
 // tested first, so that only unset bits take the (same-word serialised)
 // atomic; a stride's bit reads all go out before the first test.
-// Q32: B has fewer than 2^30 nonzeros, so a column's byte offset fits 32
-// bits and the load takes the scalar base plus a 32-bit vector offset.
-template <int T, bool Q32>
-__device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, const SegB& seg,
+// Q32: A and B have fewer than 2^30 nonzeros and B fewer than 2^28 rows,
+// so every byte offset into A's columns and B's row pointers and columns
+// fits 32 bits and each load takes the scalar base plus a 32-bit vector
+// offset.
+template <int T, bool Q32, class Seg>
+__device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, const Seg& seg,
                                                    const Csr& B, uint32_t* bm, int64_t base, int32_t pw) {
   block_expand_batch<T, kUSymBitmap>(
       p0, p1, L, seg,
@@ -884,7 +900,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
   __shared__ ExpandLds<T> L;
   int32_t start, cnt;
   bin_range(counts, bin, start, cnt);
-  const SegB seg{A.col, B.rp};
+  const std::conditional_t<Q32, SegB32, SegB> seg{A.col, B.rp};
   const int wchunk = W >> 5;
   // the pieces of heavy rows first (they are the longest work items): each
   // ORs its bitmap into the row's global bitmap, word by non-zero word
@@ -905,7 +921,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       __syncthreads();
       const int64_t base = lo + c0;
       symb_bitmap_expand<T, Q32>(q0, q1, L, seg, B, bm, base, pw);
-      for (int w = threadIdx.x; w < words; w += T) {
+      // the thread index passes through an empty asm statement, so that the
+      // compiler cannot hoist gbm + 4 * threadIdx.x out of the loops: held
+      // across every row's product loop, that address spilled
+      int tid = (int)threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      for (int w = tid; w < words; w += T) {
         const uint32_t x = bm[w];
         if (x) atomicOr(&g[(c0 >> 5) + w], x);
       }
@@ -1372,12 +1393,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
   // are loaded while the current row is expanded, sorted and compressed.
   // Row bounds travel as one dword per lane (lanes 0-3 C's row pointers,
   // 4-7 A's, 8-9 the row's products), read out by readlane.
+  // (the lane index goes through an empty asm statement, so that the three
+  // per-lane addresses are formed at each call rather than held, and
+  // spilled, across the kernel)
   auto meta_load = [&](int32_t r) -> uint32_t {
+    int ll = l;
+    asm volatile("" : "+v"(ll));
     uint32_t v = 0;
-    if (r >= 0 && l < 10) {
-      const uint32_t* src = l < 4 ? (const uint32_t*)(rp_c + r) + l
-                                  : (l < 8 ? (const uint32_t*)(A.rp + r) + (l - 4)
-                                           : (const uint32_t*)(row_flp + r) + (l - 8));
+    if (r >= 0 && ll < 10) {
+      const uint32_t* src = ll < 4 ? (const uint32_t*)(rp_c + r) + ll
+                                   : (ll < 8 ? (const uint32_t*)(A.rp + r) + (ll - 4)
+                                             : (const uint32_t*)(row_flp + r) + (ll - 8));
       v = *src;
     }
     return v;
@@ -1405,6 +1431,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
     }
   }
   for (; idx < cnt_rows; idx += gridDim.x) {
+    // the thread index passes through an empty asm statement once per row,
+    // so that the compiler cannot hold (array + thread offset) addresses
+    // across the whole kernel: they spilled, and each reload waited for
+    // every load in flight, the next row's prefetch included
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int64_t off = rd64(mv, 0);
     const int32_t z = (int32_t)(rd64(mv, 2) - off);
     const int64_t p0 = rd64(mv, 4), p1 = rd64(mv, 6);
@@ -1422,7 +1454,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
       int32_t len = plen;
       V a = pa;
       if (w0 != p0) {  // windows after the first (rows of more than T A nonzeros)
-        const int64_t p = w0 + threadIdx.x;
+        const int64_t p = w0 + tid;
         st = 0;
         len = 0;
         a = V(0);
@@ -1508,7 +1540,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
     pa = V(0);
     pst = 0;
     plen = 0;
-    const int64_t q0 = rd64(nmv, 4) + threadIdx.x;
+    const int64_t q0 = rd64(nmv, 4) + tid;
     const bool pn = q0 < rd64(nmv, 6);
     if (pn) {
       pk = A.col[q0];
@@ -1601,7 +1633,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
       }
     }
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < z; i += T) {
+    for (int32_t i = tid; i < z; i += T) {
       col_c[off + i] = (int32_t)key[i];
       val_c[off + i] = vals[i];
     }

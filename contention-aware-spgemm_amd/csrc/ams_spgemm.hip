@@ -494,7 +494,7 @@ template <int T>
 static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
                               int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords,
                               int wave_rows) {
-  const bool q32 = B.nnz < ((int64_t)1 << 30);
+  const bool q32 = A.nnz < ((int64_t)1 << 30) && B.nnz < ((int64_t)1 << 30) && B.m < ((int64_t)1 << 28);
   if constexpr (T == 1024) {
     if ((int64_t)(Wb >> 5) + Wb / W + 1 <= kSymStaticWords)
       return q32 ? launch_symb_bitmap_q<T, true, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords,
