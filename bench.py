@@ -45,8 +45,8 @@ WORKLOADS = {
                                  text="G500 s19 ef32 with PaRMAT's construction (sorted, undirected, "
                                       "loop/dup-free, 8 columns of squares), mirrored, FP64 1.0, C = A*A^T"),
     "wide_sparse": dict(kind="wide",
-                        text="uniform random A 65536 x 65536 (128 a row) times B 65536 x 2^24 (32 a row), "
-                             "FP64 1.0: rows of about 4096 products over 2^24 columns"),
+                        text="uniform random A 65536 x 65536 (128 a row) times B 65536 x 2^24 (64 a row), "
+                             "FP64 1.0: rows of about 8192 products over 2^24 columns (global-hash bins)"),
     "join_agg": dict(kind="join",
                      text="join-aggregate incidence 850000 x 20000 (lognormal(2,1.5) degrees capped 5000, "
                           "Zipf(1.2) columns), FP64 1.0, C = A^T*A"),
@@ -507,7 +507,7 @@ def main():
         log(f"fwd+bwd: {extras['fwd_bwd']}")
         extras["mtx_io"] = bench_mtx_io(16, 16)
         log(f"mtx io: {extras['mtx_io']}")
-        for name in ("er_s20_ef16", "join_agg", "parmat_g500_s19_ef32"):
+        for name in ("er_s20_ef16", "join_agg", "parmat_g500_s19_ef32", "wide_sparse"):
             if name != args.workload:
                 extras[name] = bench_workload(name, 5 if name != "parmat_g500_s19_ef32" else 3)
                 log(f"{name}: {extras[name]}")

@@ -150,7 +150,7 @@ def batch_problems(count=512, seed=0, scales=(8, 14), efs=(2, 4, 8, 16, 32)):
             for i, (s, e) in enumerate(zip(sc, ef))]
 
 
-def wide_sparse(m=65536, k=65536, n=1 << 24, deg_a=128, deg_b=32, seed=0):
+def wide_sparse(m=65536, k=65536, n=1 << 24, deg_a=128, deg_b=64, seed=0):
     """A (m x k, about deg_a nonzeros a row) and B (k x n, about deg_b a row)
     with uniform random columns, values 1.0: rows of C = A B have about
     deg_a * deg_b products spread over n columns, so they are long for an

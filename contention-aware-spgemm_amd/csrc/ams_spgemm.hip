@@ -1027,8 +1027,13 @@ static int masked_bwd_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
 extern "C" {
 
 void ams_default_params(ams_params_t* p) {
-  p->sym_hash_max = 2730;
-  p->num_hash_max = 2730;
+  // LDS tables up to two-thirds full: rows of up to 5461 products (distinct
+  // columns) take an 8192-slot table.  Against 2730 (a third full): G500
+  // s19 ef32 391.2 -> 388.7 ms, PaRMAT s19 ef32 382.8 -> 380.0 ms, ER s20
+  // ef16 and the join-aggregate product unchanged, 2^24-column rows of
+  // about 4096 products 26.5 -> 8.6 ms (an LDS table instead of a global one)
+  p->sym_hash_max = 5461;
+  p->num_hash_max = 5461;
   p->esc_max = 4096;
   p->acc_thres = 2.0;
   p->dense_width = 8448;
@@ -1039,10 +1044,13 @@ void ams_default_params(ams_params_t* p) {
   p->wgs_per_cu = 0;
   p->esc_sort = 0;
   p->sym_blocked = 0;
-  // rows of fewer products than a quarter of their range's words: G500 s19
-  // ef32 unchanged (390.4 against 391.4 ms), 2^24-column rows of about 4096
-  // products 765 -> 26 ms; 1.0 would send G500 rows that dense chunks do
-  // 2.8 times faster to the global hash
+  // rows too long for an LDS table with fewer products than a quarter of
+  // their range's words: 2^24-column rows of about 4096 products (tables
+  // then a third full) 764 -> 26 ms against dense chunks, G500 s19 ef32
+  // unchanged (391.2 against 391.5 ms); 1.0 sends G500 rows that dense
+  // chunks do 2.8 times faster to the global hash (497 ms), and every wide
+  // row of the join-aggregate product goes 35 times slower there (740
+  // against 21 ms)
   p->ghash_ratio = 0.25;
 }
 
