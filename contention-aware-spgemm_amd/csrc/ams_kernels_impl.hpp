@@ -1357,6 +1357,102 @@ __device__ __forceinline__ void block_radix_sort(uint32_t (&kr)[IPT], int f, int
   }
 }
 
+// Sorts the f keys key[0, f) of one row, made of k ascending runs (run c is
+// key[roff[c], roff[c + 1]), roff[k] = f), by rounds of pairwise merges:
+// round L merges groups of 2^L runs two by two.  Thread i writes the sorted
+// positions [i IPT, i IPT + IPT): it finds how many keys of each of its two
+// groups come before its first position by a binary search along the merge
+// path, merges IPT keys from there into registers and, after a barrier,
+// stores them.  Keys are distinct (the rank is in their low bits).  The work
+// follows f and the number of runs instead of the power of two a sorting
+// network needs, and it uses that B's rows arrive sorted.  Returns false,
+// with key[] untouched, when some run does not ascend (B rows with unsorted
+// columns); otherwise out[r] is sorted key i IPT + r (0xffffffff past f) and
+// key[] holds the sorted keys.  All threads call it: it holds barriers.
+template <int T, int IPT>
+__device__ __forceinline__ bool block_merge_runs(uint32_t* key, const uint16_t* roff, int k, int f,
+                                                 uint32_t (&out)[IPT]) {
+  const int i0 = (int)threadIdx.x * IPT;
+  int c = 0;  // run of position i0
+  bool bad = false;
+  if (i0 < f) {
+    int lo = 0, hi = k;  // roff[lo] <= i0 < roff[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)roff[mid] <= i0) lo = mid;
+      else hi = mid;
+    }
+    c = lo;
+    // this thread's keys and the next thread's first ascend inside each run
+    int cc = c + 1, nst = roff[cc];
+    uint32_t prev = key[i0];
+    for (int p = i0 + 1; p <= i0 + IPT && p < f; ++p) {
+      const uint32_t v = key[p];
+      if (p == nst) nst = roff[++cc];  // p < f = roff[k], so cc < k here
+      else bad |= !(prev < v);
+      prev = v;
+    }
+  }
+  if (__syncthreads_or(bad)) return false;
+  for (int L = 0; (1 << L) < k; ++L) {
+    if (i0 < f) {
+      // the pair of groups holding position i0: runs [a0, a1) and [a1, b1)
+      int a0 = (c >> (L + 1)) << (L + 1);
+      int a1 = a0 + (1 << L) < k ? a0 + (1 << L) : k;
+      int b1 = a0 + (2 << L) < k ? a0 + (2 << L) : k;
+      int ae = roff[a1], be = roff[b1];
+      const int as = roff[a0];
+      const int d = i0 - as;  // outputs of the pair before i0
+      int lo = d - (be - ae) > 0 ? d - (be - ae) : 0, hi = d < ae - as ? d : ae - as;
+      while (lo < hi) {  // keys of the first group among them
+        const int mid = (lo + hi) >> 1;
+        if (key[as + mid] < key[ae + d - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+      }
+      int ia = as + lo, jb = ae + d - lo;
+      uint32_t ka = ia < ae ? key[ia] : 0u, kb = jb < be ? key[jb] : 0u;
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) {
+        out[r] = 0xffffffffu;
+        if (i0 + r < f) {
+          if (ia == ae && jb == be) {  // the pair ends at roff[b1] = i0 + r < f: the next one starts
+            a0 = b1;
+            a1 = a0 + (1 << L) < k ? a0 + (1 << L) : k;
+            b1 = a0 + (2 << L) < k ? a0 + (2 << L) : k;
+            ia = roff[a0];
+            ae = roff[a1];
+            jb = ae;
+            be = roff[b1];
+            ka = key[ia];
+            kb = jb < be ? key[jb] : 0u;
+          }
+          const bool ta = ia < ae && (jb >= be || ka < kb);
+          out[r] = ta ? ka : kb;
+          const int nx = (ta ? ia : jb) + 1;
+          const uint32_t v = nx < (ta ? ae : be) ? key[nx] : 0u;
+          if (ta) {
+            ia = nx;
+            ka = v;
+          } else {
+            jb = nx;
+            kb = v;
+          }
+        }
+      }
+    }
+    __syncthreads();  // every read of this round is done
+#pragma unroll
+    for (int r = 0; r < IPT; ++r)
+      if (i0 + r < f) key[i0 + r] = out[r];
+    __syncthreads();
+  }
+  if (k <= 1) {
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) out[r] = i0 + r < f ? key[i0 + r] : 0xffffffffu;
+  }
+  return true;
+}
+
 #ifndef AMS_ESC_GROUP
 #define AMS_ESC_GROUP 4
 #endif
@@ -1365,8 +1461,14 @@ constexpr int kEscGroup = AMS_ESC_GROUP;  // products per thread in flight in th
 #ifndef AMS_ESC_WPE
 #define AMS_ESC_WPE 6
 #endif
-template <int E, int T, typename V, bool RADIX>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))) void k_numc_esc32(
+// SORT: 0 register bitonic network, 2 block LSD radix sort of the column
+// bits, 3 merge of the row's B segments (rows of one window of A nonzeros
+// whose segments ascend; other rows take the bitonic network).
+#ifndef AMS_ESC_WPE_MERGE
+#define AMS_ESC_WPE_MERGE 5
+#endif
+template <int E, int T, typename V, int SORT>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? AMS_ESC_WPE_MERGE : AMS_ESC_WPE))) void k_numc_esc32(
     Csr A, Csr B, const int32_t* list, const int32_t* counts, int bin, const int64_t* row_flp, const int64_t* rp_c,
     int32_t* col_c, V* val_c, int64_t cap, int col_bits, int32_t* err) {
   constexpr int IPT = E / T;
@@ -1378,7 +1480,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
   __shared__ __align__(16) unsigned char region[kRegion];
   __shared__ V vals[E];
   __shared__ int64_t s_w[T / kWave];
+  constexpr bool RADIX = SORT == 2, MERGE = SORT == 3;
   __shared__ int32_t s_cnt[RADIX ? 2 * T : 1];
+  __shared__ uint16_t roff[MERGE ? T + 1 : 1];  // run starts of a one-window row (merge)
+  __shared__ int32_t s_nrun;
   uint32_t* key = (uint32_t*)region;
   SegBias<V>* sb = (SegBias<V>*)region;
   uint2* wpre = (uint2*)(region + 16 * T);
@@ -1475,6 +1580,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
       if (len > 0) {
         sb[ci] = SegBias<V>{st - o, a};
         atomicOr(&wpre[o >> 5].x, 1u << (o & 31));
+        if constexpr (MERGE) roff[ci] = (uint16_t)o;
+      }
+      if constexpr (MERGE) {
+        if (threadIdx.x == 0) {
+          s_nrun = (int32_t)(tot64 >> 32);
+          roff[(int32_t)(tot64 >> 32)] = (uint16_t)total;
+        }
       }
       __syncthreads();
       if (w == 0) {  // set bits before each word
@@ -1548,22 +1660,39 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_ESC_WPE))
     }
     __syncthreads();  // products in LDS; the region is free for the sort
     const int i0 = threadIdx.x * IPT;
-    if constexpr (RADIX) {
-      block_radix_sort<T, IPT>(kr, f, RB, col_bits, key, s_cnt, s_w);
-    } else {
-      reg_bitonic_sort<T, IPT>(kr, key);
+    bool merged = false;
+    if constexpr (MERGE) {
+      if (p1 - p0 <= T) {  // one window: the runs are its B segments, keys go to their ranks
+        // every slot, without a branch (slots past f hold 0xffffffff and
+        // land past f): branches made the compiler reload the base address
+        // from scratch before each store, each reload waiting for the next
+        // row's loads in flight
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) key[(w * IPT + r) * kWave + l] = kr[r];
+        __syncthreads();
+        merged = block_merge_runs<T, IPT>(key, roff, __builtin_amdgcn_readfirstlane(s_nrun), f, kr);
+      }
+    }
+    if (!merged) {
+      if constexpr (RADIX) {
+        block_radix_sort<T, IPT>(kr, f, RB, col_bits, key, s_cnt, s_w);
+      } else {
+        reg_bitonic_sort<T, IPT>(kr, key);
+      }
     }
     if (pn) {  // and their B segments, while the runs are compressed
       pst = B.rp[pk];
       plen = (int32_t)(B.rp[pk + 1] - pst);
     }
-    if constexpr (RADIX) {  // this thread's IPT consecutive sorted keys
+    if (!merged) {
+      if constexpr (RADIX) {  // this thread's IPT consecutive sorted keys
 #pragma unroll
-      for (int r = 0; r < IPT; ++r) kr[r] = i0 + r < f ? key[i0 + r] : 0xffffffffu;
-    } else {
+        for (int r = 0; r < IPT; ++r) kr[r] = i0 + r < f ? key[i0 + r] : 0xffffffffu;
+      } else {
 #pragma unroll
-      for (int r = 0; r < IPT; ++r) key[i0 + r] = kr[r];
-      __syncthreads();
+        for (int r = 0; r < IPT; ++r) key[i0 + r] = kr[r];
+        __syncthreads();
+      }
     }
     // heads of runs of equal columns; the previous thread's last key from LDS
     const uint32_t pcol = (i0 > 0 && i0 <= f) ? (key[i0 - 1] >> RB) : 0u;

@@ -782,23 +782,22 @@ static void launch_numc_esc_k(ams_ctx_t* ctx, const Csr& A, const Csr& B, const 
   constexpr int T = hash_threads<E>();
   if constexpr (K32) {
     if (ctx->prm.esc_sort != 1) {
-      // register bitonic sort (esc_sort 0) or LSD radix sort by the column bits (2)
+      // register bitonic sort (esc_sort 0), LSD radix sort by the column bits
+      // (2) or merge of the B segments (3)
       const int col_bits = B.n > 1 ? 64 - __builtin_clzll((unsigned long long)(B.n - 1)) : 1;
-      Ctrl* c = dctrl(ctx);
-      int occ;
-      if (ctx->prm.esc_sort == 2) {
-        occupancy(k_numc_esc32<E, T, V, true>, T, 0, &occ);
+      auto go = [&](auto sort_c) {
+        constexpr int SORT = decltype(sort_c)::value;
+        Ctrl* c = dctrl(ctx);
+        int occ;
+        occupancy(k_numc_esc32<E, T, V, SORT>, T, 0, &occ);
         const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
-        hipLaunchKernelGGL((k_numc_esc32<E, T, V, true>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+        hipLaunchKernelGGL((k_numc_esc32<E, T, V, SORT>), dim3(g), dim3(T), 0, ctx->stream, A, B,
                            (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
                            (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, col_bits, &c->err);
-      } else {
-        occupancy(k_numc_esc32<E, T, V, false>, T, 0, &occ);
-        const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
-        hipLaunchKernelGGL((k_numc_esc32<E, T, V, false>), dim3(g), dim3(T), 0, ctx->stream, A, B,
-                           (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
-                           (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, col_bits, &c->err);
-      }
+      };
+      if (ctx->prm.esc_sort == 2) go(std::integral_constant<int, 2>{});
+      else if (ctx->prm.esc_sort == 3) go(std::integral_constant<int, 3>{});
+      else go(std::integral_constant<int, 0>{});
       return;
     }
   }
@@ -1129,7 +1128,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
     return fail(ctx, AMS_ERR_ARG, "dense_width must be a multiple of 64 in [64, 65536]");
   if (p->bitmap_bits < p->dense_width) return fail(ctx, AMS_ERR_ARG, "bitmap_bits must be >= dense_width");
   if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
-  if (p->esc_sort < 0 || p->esc_sort > 2) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0, 1 or 2");
+  if (p->esc_sort < 0 || p->esc_sort > 3) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0, 1, 2 or 3");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
   ctx->prm = *p;
@@ -1655,7 +1654,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   for (double t : thres) { ams_params_t p = best; p.acc_thres = t; consider(p); }
   for (double q : ratios) { ams_params_t p = best; p.bitmap_ratio = q; consider(p); }
   for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
-  for (int e = 0; e < 3; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
+  for (int e = 0; e < 4; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
   for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.sym_blocked = e; consider(p); }
   for (double g : ghash) { ams_params_t p = best; p.ghash_ratio = g; consider(p); }
   (void)hipEventDestroy(e0);

@@ -33,6 +33,7 @@ ROUTES = {
                               bitmap_bits=1024, dense_density=0.0),
     "esc_bucket": dict(mode=2, esc_sort=1),
     "esc_radix": dict(mode=2, esc_sort=2),
+    "esc_merge": dict(mode=2, esc_sort=3),
     "ghash": dict(sym_hash_max=64, num_hash_max=64, ghash_ratio=1e9, mode=1),
     "ghash_hybrid": dict(sym_hash_max=64, num_hash_max=64, ghash_ratio=1e9),
 }
@@ -73,7 +74,8 @@ def test_rmat_square(gpu_ctx, route, name, a, exact):
     assert_matches_oracle(c, a, a, exact)
 
 
-@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only", "blocked_sym_dense", "esc_bucket", "ghash"])
+@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only", "blocked_sym_dense", "esc_bucket", "esc_merge",
+                                   "ghash"])
 def test_rectangular_unsorted(gpu_ctx, route):
     a = random_csr(70, 300, 0.05, seed=1)
     b = random_csr(300, 2000, 0.02, seed=2, sort=False)
@@ -157,7 +159,7 @@ def test_wide_bitmap_dynamic_lds(gpu_ctx):
     assert_matches_oracle(c, a, b, exact=True)
 
 
-@pytest.mark.parametrize("route", ["default", "esc_only"])
+@pytest.mark.parametrize("route", ["default", "esc_only", "esc_merge"])
 @pytest.mark.parametrize("b_density", [0.0005, 0.002, 0.0055])
 def test_esc_long_a_rows(gpu_ctx, route, b_density):
     """ESC rows with 260+ A nonzeros over short B rows: several expansion
@@ -302,7 +304,7 @@ ESC_CASES = [("rmat_s8_ef4", lambda: gen.rmat_symmetric(8, 4, seed=101, values="
              ("rect_unsorted", None)]
 
 
-@pytest.mark.parametrize("esc_sort", [0, 1, 2])
+@pytest.mark.parametrize("esc_sort", [0, 1, 2, 3])
 @pytest.mark.parametrize("name,make", ESC_CASES, ids=[c[0] for c in ESC_CASES])
 def test_esc_random_values_bit_exact(gpu_ctx, name, make, esc_sort):
     """Random FP64 values through the ESC route: every row of at most esc_max
