@@ -152,7 +152,7 @@ TUNE_SPACE = [
     ("acc_thres", [0.0, 1.25, 2.0, 4.0]),
     ("bitmap_ratio", [0.25, 1.0, 4.0]),
     ("dense_density", [0.02, 0.05, 0.2, 1.0]),
-    ("esc_sort", [0, 1, 2, 3]),
+    ("esc_sort", [0, 1, 2, 3, 4]),
 ]
 
 

@@ -498,6 +498,120 @@ __device__ __forceinline__ void reg_bitonic_sort(K (&v)[IPT], K* s) {
   bitonic_levels<T, IPT, 2>(v, s, (int)threadIdx.x * IPT);
 }
 
+// Lane i ^ M's value for M = 2^k - 1 (the mirror partner inside groups of
+// M + 1 lanes): quad_perm [1,0,3,2], quad_perm [3,2,1,0], row_half_mirror
+// and row_mirror are one DPP move each; 31 and 63 go through ds_bpermute.
+template <int M>
+__device__ __forceinline__ uint32_t lane_mirror(uint32_t x) {
+  if constexpr (M == 1) {
+    return dpp_move<0xb1, 0xf>(x);
+  } else if constexpr (M == 3) {
+    return dpp_move<0x1b, 0xf>(x);
+  } else if constexpr (M == 7) {
+    return dpp_move<0x141, 0xf>(x);
+  } else if constexpr (M == 15) {
+    return dpp_move<0x140, 0xf>(x);
+  } else {
+    return (uint32_t)__shfl_xor((int)x, M, kWave);
+  }
+}
+
+// One stage of the flip (mirror) bitonic network on T*IPT 32-bit keys held
+// IPT per thread, thread-contiguous (element e = base + r in v[r], base =
+// tid*IPT).  MIR: the first stage of merge size KS, partner e ^ (KS - 1);
+// otherwise a half-cleaner, partner e ^ J.  The lower element of a pair
+// always takes the minimum, so no stage has a direction.  Elements at and
+// past nact are padding (0xffffffff) in waves that hold nothing else; such
+// a wave (act false) skips the stage, and an element whose partner lies
+// there keeps its key, which is what the comparator would leave (the
+// partner is the higher element and the larger key).  Every wave still
+// meets the barriers of the stages that cross waves through LDS.
+template <int T, int IPT, int KS, int J, bool MIR>
+__device__ __forceinline__ void flip_stage(uint32_t (&v)[IPT], uint32_t* s, int base, bool act, int nact) {
+  if constexpr (MIR ? KS <= IPT : J < IPT) {  // inside the thread's registers
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) {
+        const int p = MIR ? (r ^ (KS - 1)) : (r ^ J);
+        if (r < p) {
+          const uint32_t lo = min(v[r], v[p]), hi = max(v[r], v[p]);
+          v[r] = lo;
+          v[p] = hi;
+        }
+      }
+    }
+  } else if constexpr (MIR ? KS <= kWave * IPT : J < kWave * IPT) {  // lanes of the same wave
+    if (act) {
+      uint32_t o[IPT];
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) {
+        if constexpr (MIR) o[r] = lane_mirror<KS / IPT - 1>(v[IPT - 1 - r]);
+        else o[r] = lane_xor<J / IPT>(v[r]);
+      }
+      const bool lower = (lane_id() & (MIR ? KS / (2 * IPT) : J / IPT)) == 0;
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) v[r] = lower ? min(v[r], o[r]) : max(v[r], o[r]);
+    }
+  } else {  // another wave's registers: through LDS
+    // The partner of every element of wave w lies in one wave, w ^ (KS/WK - 1)
+    // (mirror) or w ^ (J/WK); a partner wave in the padding suffix leaves
+    // this wave's keys as they are.
+    constexpr int WK = kWave * IPT;
+    const int pw = MIR ? (wave_id() ^ (KS / WK - 1)) : (wave_id() ^ (J / WK));
+    const bool both = act && __builtin_amdgcn_readfirstlane(pw * WK) < nact;
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) s[base + r] = v[r];
+    }
+    __syncthreads();
+    if (both) {
+      uint32_t o[IPT];  // every partner read goes out before any compare
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) o[r] = s[MIR ? ((base + r) ^ (KS - 1)) : ((base + r) ^ J)];
+#pragma unroll
+      for (int r = 0; r < IPT; ++r) {
+        const int e = base + r, p = MIR ? (e ^ (KS - 1)) : (e ^ J);
+        v[r] = e < p ? min(v[r], o[r]) : max(v[r], o[r]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int T, int IPT, int KS, int J>
+__device__ __forceinline__ void flip_halves(uint32_t (&v)[IPT], uint32_t* s, int base, bool act, int nact) {
+  if constexpr (J >= 1) {
+    flip_stage<T, IPT, KS, J, false>(v, s, base, act, nact);
+    flip_halves<T, IPT, KS, J / 2>(v, s, base, act, nact);
+  }
+}
+
+constexpr int next_pow2_c(int x) { return x <= 1 ? 1 : 2 * next_pow2_c((x + 1) / 2); }
+
+template <int T, int IPT, int KS>
+__device__ __forceinline__ void flip_levels(uint32_t (&v)[IPT], uint32_t* s, int base, bool act, int nact) {
+  flip_stage<T, IPT, KS, KS / 2, true>(v, s, base, act, nact);
+  flip_halves<T, IPT, KS, KS / 4>(v, s, base, act, nact);
+  if constexpr (KS < next_pow2_c(T * IPT)) flip_levels<T, IPT, KS * 2>(v, s, base, act, nact);
+}
+
+// Ascending sort of T*IPT 32-bit keys, IPT per thread, thread-contiguous,
+// by the flip bitonic network over the next power of two of T*IPT keys (the
+// positions past T*IPT are virtual padding), of which only the first f
+// elements (in whole waves of 64*IPT) take part: the rest must be
+// 0xffffffff, and those waves skip every stage.  s: T*IPT keys of LDS
+// (stages that cross waves).  All threads call it: it holds barriers when
+// T > 64.
+template <int T, int IPT>
+__device__ __forceinline__ void reg_flip_sort(uint32_t (&v)[IPT], uint32_t* s, int f) {
+  static_assert(IPT == 1 || IPT == 2 || IPT == 4 || IPT == 8, "1, 2, 4 or 8 keys per thread");
+  static_assert(T * IPT >= 2 && T % kWave == 0, "whole waves, at least two keys");
+  constexpr int kWaveKeys = kWave * IPT;
+  const int nact = (f + kWaveKeys - 1) / kWaveKeys * kWaveKeys;
+  const bool act = __builtin_amdgcn_readfirstlane(wave_id() * kWaveKeys) < nact;
+  flip_levels<T, IPT, 2>(v, s, (int)threadIdx.x * IPT, act, nact);
+}
+
 // Sorts perm[0..n) = 0..n-1 by the composite (key[i], i) ascending: a
 // counting pass over nb (power of two, >= n) order-preserving buckets of
 // the key range [kmin, kmin + span), a scatter, and an insertion sort of

@@ -25,12 +25,21 @@ enum : uint8_t { SB_NONE = 0xff, SB_BITMAP = 8, SB_GHASH = 9 };  // 0..7: LDS ha
 constexpr int kSymHashBins = 8;
 constexpr int kSymBins = 10;
 // Numeric routes.
-enum : uint8_t { NB_NONE = 0xff, NB_DENSE = 16, NB_GHASH = 17 };  // 0..7 hash (64 << b), 8..14 ESC (64 << (b-8))
+// 0..7 LDS hash (table 64 << b); 8..16 ESC, products per row at most
+// esc_bin_cap(b - 8): 64, 128, 256, 512, 1024, 1536, 2048, 3072, 4096
+enum : uint8_t { NB_NONE = 0xff, NB_DENSE = 17, NB_GHASH = 18 };
 constexpr int kNumHashBins = 8;
-constexpr int kNumEscBins = 7;
-constexpr int kNumBins = 18;
+constexpr int kNumEscBins = 9;
+constexpr int kNumBins = 19;
 constexpr int kMaxHashSlots = 64 << (kNumHashBins - 1);  // 8192
-constexpr int kMaxEsc = 64 << (kNumEscBins - 1);         // 4096
+constexpr int kMaxEsc = 4096;
+__host__ __device__ constexpr int esc_bin_cap(int i) { return i <= 4 ? 64 << i : (i & 1 ? 1536 : 2048) << ((i - 5) >> 1); }
+// ESC bin of a row of f products, 1 <= f <= kMaxEsc
+__host__ __device__ inline int esc_bin_of(int64_t f) {
+  int i = 0;
+  while (esc_bin_cap(i) < f) ++i;
+  return i;
+}
 
 struct Csr {
   int64_t m, n, nnz;

@@ -427,8 +427,7 @@ __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* 
           b = NB_DENSE;
         } else if (np.mode != 1 && f <= np.esc_max &&
                    (np.mode == 2 || (double)f / (double)z < np.acc_thres)) {
-          const int64_t e = next_pow2_i64(f);
-          b = kNumHashBins + (e <= 64 ? 0 : ilog2_pow2(e) - 6);
+          b = kNumHashBins + esc_bin_of(f);
         } else if (z <= np.num_hash_max) {
           const int64_t s = next_pow2_i64(z + ((z + 1) >> 1));
           b = s <= 64 ? 0 : ilog2_pow2(s) - 6;
@@ -1357,6 +1356,140 @@ __device__ __forceinline__ void block_radix_sort(uint32_t (&kr)[IPT], int f, int
   }
 }
 
+// Sorts the f keys of one row (column << lo | rank: distinct, one per
+// product slot i < f, slot i of this thread in kr[s] with
+// i = (wave * IPT + s) * 64 + lane) into buf[0, f) without a sorting
+// network.  A counting pass spreads the keys over NB = T * IPT buckets by an
+// order-preserving map of the row's column range (every key of a bucket is
+// smaller than every key of the next one, equal columns share a bucket), a
+// scan turns the counts into starts, the keys are scattered, and each key's
+// place is its bucket's start plus the keys of its bucket that are smaller
+// than it.  Rows of uniform graphs leave about f / NB keys a bucket, so a
+// key costs a few LDS operations instead of the log2(E)(log2(E)+1)/2 stages
+// of the bitonic network over the padded bin.  Returns false, with kr
+// untouched and buf clobbered, when some bucket holds more than
+// kEscBucketMax keys (clustered columns); the caller then sorts kr itself.
+// buf: NB words; s_w: T / 64 int64 of scratch.  Holds barriers; every thread
+// of the block calls it.
+constexpr int kEscBucketMax = 16;
+template <int T, int IPT>
+__device__ __forceinline__ bool block_bucket_sort(const uint32_t (&kr)[IPT], int f, int lo, uint32_t* buf,
+                                                  int64_t* s_w) {
+  constexpr int NW = T / kWave;
+  constexpr int NB = T * IPT;
+  static_assert(NB <= 4096, "12-bit bucket indices and positions");
+  const int w = wave_id(), l = lane_id();
+  uint32_t cmin = 0xffffffffu, cmax = 0u;
+#pragma unroll
+  for (int s = 0; s < IPT; ++s) {
+    if ((w * IPT + s) * kWave + l < f) {
+      const uint32_t c = kr[s] >> lo;
+      cmin = min(cmin, c);
+      cmax = max(cmax, c);
+    }
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    cmin = min(cmin, (uint32_t)__shfl_xor((int)cmin, d, kWave));
+    cmax = max(cmax, (uint32_t)__shfl_xor((int)cmax, d, kWave));
+  }
+  uint32_t* s_mm = (uint32_t*)s_w;  // (min, max) of each wave
+  if (l == 0) {
+    s_mm[2 * w] = cmin;
+    s_mm[2 * w + 1] = cmax;
+  }
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) buf[threadIdx.x * IPT + q] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NW; ++v) {
+    cmin = min(cmin, s_mm[2 * v]);
+    cmax = max(cmax, s_mm[2 * v + 1]);
+  }
+  // bucket = floor((column - cmin) * NB / span), computed in float: the
+  // conversion, the product with a positive scale and the truncation are
+  // each non-decreasing, so the map keeps order
+  const float scale = (float)NB / ((float)(cmax - cmin) + 1.0f);
+  auto bucket = [&](uint32_t k) -> uint32_t {
+    const uint32_t b = (uint32_t)((float)((k >> lo) - cmin) * scale);
+    return b < (uint32_t)NB ? b : (uint32_t)(NB - 1);
+  };
+  // pk[s] packs a key's bucket size (bits 24-31, read only while at most
+  // kEscBucketMax), its bucket and later its scattered position (bits
+  // 12-23) and its position in the bucket from the counting atomic (0-11),
+  // so that a key costs two registers
+  uint32_t pk[IPT];
+#pragma unroll
+  for (int s = 0; s < IPT; ++s) {
+    pk[s] = 0u;
+    if ((w * IPT + s) * kWave + l < f) {
+      const uint32_t b = bucket(kr[s]);
+      pk[s] = (b << 12) | (uint32_t)atomicAdd(&buf[b], 1u);
+    }
+  }
+  __syncthreads();
+  // this thread's NB / T counters, the sizes of its keys' buckets, then the
+  // starts written over the counts (after the scan's barrier, so every size
+  // read is done)
+  uint32_t sum = 0, big = 0;
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const uint32_t c = buf[threadIdx.x * IPT + q];
+    sum += c;
+    big |= c > (uint32_t)kEscBucketMax;
+  }
+#pragma unroll
+  for (int s = 0; s < IPT; ++s)
+    if ((w * IPT + s) * kWave + l < f) pk[s] |= min(buf[pk[s] >> 12], 255u) << 24;
+  int64_t tot;
+  uint32_t run = (uint32_t)block_excl_scan_once<T, int64_t>((int64_t)sum, s_w, tot);
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const uint32_t c = buf[threadIdx.x * IPT + q];
+    buf[threadIdx.x * IPT + q] = run;
+    run += c;
+  }
+  if (__syncthreads_or((int)big)) return false;
+#pragma unroll
+  for (int s = 0; s < IPT; ++s) {
+    if ((w * IPT + s) * kWave + l < f) {
+      const uint32_t pos = buf[(pk[s] >> 12) & 0xfffu] + (pk[s] & 0xfffu);
+      pk[s] = (pk[s] & 0xff000fffu) | (pos << 12);
+    }
+  }
+  __syncthreads();  // every start is read: the keys go over the counters
+#pragma unroll
+  for (int s = 0; s < IPT; ++s)
+    if ((w * IPT + s) * kWave + l < f) buf[(pk[s] >> 12) & 0xfffu] = kr[s];
+  __syncthreads();
+  // place = bucket start + smaller keys of the bucket; a key of a bucket of
+  // one stays where it is.  (Keys are read back from their scattered
+  // slots, so that kr is dead from the scatter on.)
+  uint32_t kv[IPT];
+#pragma unroll
+  for (int s = 0; s < IPT; ++s) {
+    kv[s] = 0u;
+    if ((w * IPT + s) * kWave + l < f && (pk[s] >> 24) > 1u) {
+      const uint32_t p = (pk[s] >> 12) & 0xfffu;
+      const uint32_t st = p - (pk[s] & 0xfffu);
+      const uint32_t en = st + (pk[s] >> 24);
+      const uint32_t k = buf[p];
+      uint32_t r = st;
+      for (uint32_t j = st; j < en; ++j) r += buf[j] < k ? 1u : 0u;
+      kv[s] = k;
+      pk[s] = r | 0x80000000u;
+    } else {
+      pk[s] = 0u;
+    }
+  }
+  __syncthreads();  // every bucket read is done
+#pragma unroll
+  for (int s = 0; s < IPT; ++s)
+    if (pk[s]) buf[pk[s] & 0xfffu] = kv[s];
+  __syncthreads();
+  return true;
+}
+
 // Sorts the f keys key[0, f) of one row, made of k ascending runs (run c is
 // key[roff[c], roff[c + 1]), roff[k] = f), by rounds of pairwise merges:
 // round L merges groups of 2^L runs two by two.  Thread i writes the sorted
@@ -1463,24 +1596,35 @@ constexpr int kEscGroup = AMS_ESC_GROUP;  // products per thread in flight in th
 #endif
 // SORT: 0 register bitonic network, 2 block LSD radix sort of the column
 // bits, 3 merge of the row's B segments (rows of one window of A nonzeros
-// whose segments ascend; other rows take the bitonic network).
+// whose segments ascend; other rows take the bitonic network), 4 bucket
+// sort over the row's column range (rows with a crowded bucket take the
+// bitonic network).
 #ifndef AMS_ESC_WPE_MERGE
 #define AMS_ESC_WPE_MERGE 5
 #endif
+#ifndef AMS_ESC_WPE_COUNT
+#define AMS_ESC_WPE_COUNT 5
+#endif
+// 512-thread blocks of 49 KB of LDS: three a CU at six waves a SIMD (80
+// VGPRs, with spills), two at four (128 VGPRs, none)
+#ifndef AMS_ESC_WPE_4096
+#define AMS_ESC_WPE_4096 4
+#endif
 template <int E, int T, typename V, int SORT>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? AMS_ESC_WPE_MERGE : AMS_ESC_WPE))) void k_numc_esc32(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
+    E >= 4096 ? AMS_ESC_WPE_4096 : (SORT == 3 ? AMS_ESC_WPE_MERGE : (SORT == 4 ? AMS_ESC_WPE_COUNT : AMS_ESC_WPE))))) void k_numc_esc32(
     Csr A, Csr B, const int32_t* list, const int32_t* counts, int bin, const int64_t* row_flp, const int64_t* rp_c,
     int32_t* col_c, V* val_c, int64_t cap, int col_bits, int32_t* err) {
   constexpr int IPT = E / T;
-  constexpr int RB = __builtin_ctz(E);
+  constexpr int RB = __builtin_ctz(next_pow2_c(E));  // rank bits of a key
   constexpr int kWords = E / 32;                               // bitmap words of a window
-  constexpr int kWpl = kWords >= kWave ? kWords / kWave : 1;   // words per lane of wave 0
+  constexpr int kWpl = (kWords + kWave - 1) / kWave;           // words per lane of wave 0
   constexpr int kRegion = 4 * E > 16 * T + 8 * kWords ? 4 * E : 16 * T + 8 * kWords;
   static_assert(T * IPT == E && IPT <= 16, "the bin fills the threads' registers");
   __shared__ __align__(16) unsigned char region[kRegion];
   __shared__ V vals[E];
   __shared__ int64_t s_w[T / kWave];
-  constexpr bool RADIX = SORT == 2, MERGE = SORT == 3;
+  constexpr bool RADIX = SORT == 2, MERGE = SORT == 3, BUCKET = SORT == 4;
   __shared__ int32_t s_cnt[RADIX ? 2 * T : 1];
   __shared__ uint16_t roff[MERGE ? T + 1 : 1];  // run starts of a one-window row (merge)
   __shared__ int32_t s_nrun;
@@ -1608,7 +1752,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? A
       }
       __syncthreads();
       // this thread's products of the window, in groups of kEscGroup slots:
-      // a group's LDS reads and loads are all issued before any is used
+      // a group's LDS reads and loads are all issued before any is used.  A
+      // wave whose slots [w*IPT*64, (w+1)*IPT*64) miss [base, base + total)
+      // has nothing to expand.
+      const int32_t wslot = __builtin_amdgcn_readfirstlane(w * IPT * kWave);
+      if (wslot < base + total && wslot + IPT * kWave > base)
       static_for<0, (IPT + kEscGroup - 1) / kEscGroup>([&](auto gc) {
         constexpr int g0 = decltype(gc)::value * kEscGroup;
         constexpr int G = IPT - g0 < kEscGroup ? IPT - g0 : kEscGroup;
@@ -1673,11 +1821,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? A
         merged = block_merge_runs<T, IPT>(key, roff, __builtin_amdgcn_readfirstlane(s_nrun), f, kr);
       }
     }
+    bool in_lds = RADIX;  // sorted keys in key[0, f) rather than in the registers
     if (!merged) {
       if constexpr (RADIX) {
         block_radix_sort<T, IPT>(kr, f, RB, col_bits, key, s_cnt, s_w);
+      } else if constexpr (BUCKET) {
+        in_lds = block_bucket_sort<T, IPT>(kr, f, RB, key, s_w);
+        if (!in_lds) reg_flip_sort<T, IPT>(kr, key, f);
       } else {
-        reg_bitonic_sort<T, IPT>(kr, key);
+        reg_flip_sort<T, IPT>(kr, key, f);
       }
     }
     if (pn) {  // and their B segments, while the runs are compressed
@@ -1685,7 +1837,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? A
       plen = (int32_t)(B.rp[pk + 1] - pst);
     }
     if (!merged) {
-      if constexpr (RADIX) {  // this thread's IPT consecutive sorted keys
+      if (in_lds) {  // this thread's IPT consecutive sorted keys
 #pragma unroll
         for (int r = 0; r < IPT; ++r) kr[r] = i0 + r < f ? key[i0 + r] : 0xffffffffu;
       } else {
@@ -1723,7 +1875,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? A
       ocol[r] = 0u;
       osum[r] = V(0);
       if (i0 + r < f) {
-        const V x = vals[kr[r] & (uint32_t)(E - 1)];
+        const V x = vals[kr[r] & ((1u << RB) - 1u)];
         if ((hd >> r) & 1u) {
           if constexpr (r > 0) {
             if (open) {
@@ -1744,7 +1896,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(SORT == 3 ? A
       for (int32_t g = i0 + IPT; g < f; ++g) {
         const uint32_t kg = key[g];
         if ((kg >> RB) != rc) break;
-        sum += vals[kg & (uint32_t)(E - 1)];
+        sum += vals[kg & ((1u << RB) - 1u)];
       }
       ocol[IPT - 1] = rc;
       osum[IPT - 1] = sum;

@@ -781,7 +781,7 @@ static void launch_numc_esc_k(ams_ctx_t* ctx, const Csr& A, const Csr& B, const 
                               V* val_c, int64_t cap, int bin, int64_t cnt) {
   constexpr int T = hash_threads<E>();
   if constexpr (K32) {
-    if (ctx->prm.esc_sort != 1) {
+    {
       // register bitonic sort (esc_sort 0), LSD radix sort by the column bits
       // (2) or merge of the B segments (3)
       const int col_bits = B.n > 1 ? 64 - __builtin_clzll((unsigned long long)(B.n - 1)) : 1;
@@ -797,28 +797,34 @@ static void launch_numc_esc_k(ams_ctx_t* ctx, const Csr& A, const Csr& B, const 
       };
       if (ctx->prm.esc_sort == 2) go(std::integral_constant<int, 2>{});
       else if (ctx->prm.esc_sort == 3) go(std::integral_constant<int, 3>{});
+      else if (ctx->prm.esc_sort == 4) go(std::integral_constant<int, 4>{});
       else go(std::integral_constant<int, 0>{});
-      return;
     }
+  } else {
+    int occ;
+    occupancy(k_numc_esc<E, T, V>, T, 0, &occ);
+    const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
+    Ctrl* c = dctrl(ctx);
+    hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
+                       (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
+                       (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
   }
-  int occ;
-  occupancy(k_numc_esc<E, T, V>, T, 0, &occ);
-  const int g = grid_for(ctx, occ, cnt, ctx->plan_counts);
-  Ctrl* c = dctrl(ctx);
-  hipLaunchKernelGGL((k_numc_esc<E, T, V>), dim3(g), dim3(T), 0, ctx->stream, A, B,
-                     (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num, bin,
-                     (const int64_t*)ctx->row_flp.p, rp_c, col_c, val_c, cap, ctx->prm.esc_sort, &c->err);
 }
 
+// ESC bin of rows of at most E products.  E may be 3/4 of a power of two:
+// the 32-bit-key kernel then runs T = E/8 threads and sorts with the next
+// power of two's network, whose last quarter is virtual; the 64-bit-key
+// kernel takes such a bin with the next power of two's geometry.
 template <int E, typename V>
 static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
                             V* val_c, int64_t cap, int bin) {
   const int64_t cnt = ctx->plan_counts ? ctx->cnt_num[bin] : A.m;
   if (ctx->plan_counts && cnt == 0) return;
-  // (column, rank) fits 32 bits as column << log2(E) | rank below 2^(32 - log2(E)) columns
-  const bool key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(E))) && ctx->prm.esc_sort != 1;
+  constexpr int EP = next_pow2_c(E);
+  // (column, rank) fits 32 bits as column << log2(EP) | rank below 2^(32 - log2(EP)) columns
+  const bool key32 = (uint64_t)B.n <= (1ull << (32 - __builtin_ctz(EP))) && ctx->prm.esc_sort != 1;
   if (key32) launch_numc_esc_k<E, V, true>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
-  else launch_numc_esc_k<E, V, false>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
+  else launch_numc_esc_k<EP, V, false>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
 }
 
 template <int T, typename V>
@@ -912,8 +918,10 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
     launch_numc_esc<256, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 2);
     launch_numc_esc<512, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 3);
     launch_numc_esc<1024, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 4);
-    launch_numc_esc<2048, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 5);
-    launch_numc_esc<4096, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 6);
+    launch_numc_esc<1536, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 5);
+    launch_numc_esc<2048, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 6);
+    launch_numc_esc<3072, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 7);
+    launch_numc_esc<4096, V>(ctx, A, B, rp_c, col_c, val_c, cap, kNumHashBins + 8);
     HIPCHK(hipGetLastError());
   }
   return AMS_OK;
@@ -1128,7 +1136,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
     return fail(ctx, AMS_ERR_ARG, "dense_width must be a multiple of 64 in [64, 65536]");
   if (p->bitmap_bits < p->dense_width) return fail(ctx, AMS_ERR_ARG, "bitmap_bits must be >= dense_width");
   if (p->mode < 0 || p->mode > 2) return fail(ctx, AMS_ERR_ARG, "unknown mode %d", p->mode);
-  if (p->esc_sort < 0 || p->esc_sort > 3) return fail(ctx, AMS_ERR_ARG, "esc_sort must be 0, 1, 2 or 3");
+  if (p->esc_sort < 0 || p->esc_sort > 4) return fail(ctx, AMS_ERR_ARG, "esc_sort must be in [0, 4]");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
   ctx->prm = *p;
@@ -1654,7 +1662,7 @@ int ams_autotune(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int d
   for (double t : thres) { ams_params_t p = best; p.acc_thres = t; consider(p); }
   for (double q : ratios) { ams_params_t p = best; p.bitmap_ratio = q; consider(p); }
   for (double d : dens) { ams_params_t p = best; p.dense_density = d; consider(p); }
-  for (int e = 0; e < 4; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
+  for (int e = 0; e < 5; ++e) { ams_params_t p = best; p.esc_sort = e; consider(p); }
   for (int e = 0; e < 2; ++e) { ams_params_t p = best; p.sym_blocked = e; consider(p); }
   for (double g : ghash) { ams_params_t p = best; p.ghash_ratio = g; consider(p); }
   (void)hipEventDestroy(e0);

@@ -62,9 +62,12 @@ typedef struct {
 	double  dense_density;  /* dense bin also when nnz(C row) >= density * n          */
 	int32_t mode;           /* AMS_MODE_*                                             */
 	int32_t wgs_per_cu;     /* persistent workgroups per CU for the queue kernels     */
-	int32_t esc_sort;       /* ESC row sort: 0 register bitonic, 1 bucket, 2 LSD radix,
-	                           3 merge of the row's sorted B rows (rows of one window of
-	                           A nonzeros whose B rows ascend; others bitonic) */
+	int32_t esc_sort;       /* ESC row sort: 0 register bitonic, 1 bucket (64-bit keys),
+	                           2 LSD radix, 3 merge of the row's sorted B rows (rows of
+	                           one window of A nonzeros whose B rows ascend; others
+	                           bitonic), 4 counting sort over the row's column range with
+	                           rank fix-up per bucket (rows with a bucket of more than 16
+	                           keys: bitonic) */
 	int32_t sym_blocked;    /* wide-row symbolic per (row, column chunk) on blocked B */
 	double  ghash_ratio;    /* rows with flops > sym_hash_max and flops < ratio * words(n)
 	                           count and accumulate in global-memory hash tables

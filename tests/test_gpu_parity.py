@@ -34,6 +34,7 @@ ROUTES = {
     "esc_bucket": dict(mode=2, esc_sort=1),
     "esc_radix": dict(mode=2, esc_sort=2),
     "esc_merge": dict(mode=2, esc_sort=3),
+    "esc_count": dict(mode=2, esc_sort=4),
     "ghash": dict(sym_hash_max=64, num_hash_max=64, ghash_ratio=1e9, mode=1),
     "ghash_hybrid": dict(sym_hash_max=64, num_hash_max=64, ghash_ratio=1e9),
 }
@@ -304,7 +305,7 @@ ESC_CASES = [("rmat_s8_ef4", lambda: gen.rmat_symmetric(8, 4, seed=101, values="
              ("rect_unsorted", None)]
 
 
-@pytest.mark.parametrize("esc_sort", [0, 1, 2, 3])
+@pytest.mark.parametrize("esc_sort", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("name,make", ESC_CASES, ids=[c[0] for c in ESC_CASES])
 def test_esc_random_values_bit_exact(gpu_ctx, name, make, esc_sort):
     """Random FP64 values through the ESC route: every row of at most esc_max
