@@ -17,26 +17,58 @@
 
 namespace ams {
 
-// Wave-segmented sum of v over consecutive lanes with equal key (keys are
-// non-decreasing across the wave); returns true on the first lane of each
-// run, which then holds the run's total in v.
-template <typename V>
-__device__ __forceinline__ bool wave_run_sum(int key, V& v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const V y = __shfl_down(v, d, kWave);
-    const int ky = __shfl_down(key, d, kWave);
-    if (lane + d < kWave && ky == key) v += y;
+// Value of lane i - d inside 16-lane rows (row_shr), or of the last lane of
+// the previous row / of row 1 (row_bcast:15 / :31) for the rows in kRowMask;
+// 0 where there is no source.
+template <int kCtrl, int kRowMask, typename V>
+__device__ __forceinline__ V dpp_val(V v) {
+  if constexpr (sizeof(V) == 8) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = dpp_move<kCtrl, kRowMask>((uint32_t)u);
+    const uint32_t hi = dpp_move<kCtrl, kRowMask>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(V, ((uint64_t)hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(V, dpp_move<kCtrl, kRowMask>(__builtin_bit_cast(uint32_t, v)));
   }
-  const int kp = __shfl_up(key, 1, kWave);
-  return lane == 0 || kp != key;
 }
 
-// One window-flattened pass over the products of A nonzeros [p0, p1), like
-// block_expand, but every lane runs every iteration (so the body may use
-// wave shuffles) and `flush(j, p)` runs per nonzero after each window.
-template <int T, class Seg, class Body, class Flush>
+// Wave-segmented sum of v over runs of consecutive lanes with equal key: a
+// segmented inclusive scan by DPP lane moves (the scan of wave_incl_scan
+// with a run-start flag that stops each addition), so the last lane of each
+// run ends with the run's total; returns true on that lane.
+template <typename V>
+__device__ __forceinline__ bool wave_seg_sum(int key, V& v) {
+  const int lane = lane_id();
+  const int kp = __shfl_up(key, 1, kWave), kn = __shfl_down(key, 1, kWave);
+  uint32_t f = (lane == 0 || kp != key) ? 1u : 0u;  // a run starts in the lanes covered so far
+  auto step = [&](auto ctrl_c, auto mask_c) {
+    constexpr int C = decltype(ctrl_c)::value, M = decltype(mask_c)::value;
+    const V vs = dpp_val<C, M>(v);
+    const uint32_t fs = dpp_move<C, M>(f);
+    v = f ? v : v + vs;
+    f |= fs;
+  };
+  step(std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xf>{});  // row_shr:1
+  step(std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xf>{});  // row_shr:2
+  step(std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xf>{});  // row_shr:4
+  step(std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xf>{});  // row_shr:8
+  step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xa>{});  // row_bcast:15
+  step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xc>{});  // row_bcast:31
+  return lane == kWave - 1 || kn != key;
+}
+
+// One window-flattened pass over the products of A nonzeros [p0, p1).  In
+// every stride of T*U products of a window, wave w owns the 64*U consecutive
+// products from w*64*U and lane l takes products l, l + 64, ... of them, so
+// each group of 64 (one per u) covers 64 consecutive products, over which
+// the body may run wave-wide operations (every lane runs every u).  Each
+// lane keeps the segment of its last product and seeks forward from it.
+//   seg(p, st&, len&)          segment of nonzero p, on thread p - w0
+//   body(ok[U], j[U], q[U])    j = index of the nonzero in the window, q =
+//                              B position (lanes past the end: ok false, q
+//                              and j those of the window's last product)
+//   flush(j, p)                per nonzero after each window
+template <int T, int U, class Seg, class Body, class Flush>
 __device__ __forceinline__ void block_expand_flush(int64_t p0, int64_t p1, ExpandLds<T>& L, Seg seg, Body body,
                                                    Flush flush) {
   for (int64_t w0 = p0; w0 < p1; w0 += T) {
@@ -51,24 +83,36 @@ __device__ __forceinline__ void block_expand_flush(int64_t p0, int64_t p1, Expan
     __syncthreads();
     const int32_t total = (int32_t)total64;
     const int nvalid = (int)((p1 - w0) < T ? (p1 - w0) : T);
-    for (int32_t t0 = 0; t0 < total; t0 += T) {
-      const int32_t t = t0 + threadIdx.x;
-      const bool valid = t < total;
-      int lo = 0;
-      if (valid) {
-        int hi = nvalid;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (L.off[mid] <= t) lo = mid; else hi = mid;
+    int cur = 0;
+    int32_t seg_end = -1;
+    int64_t seg_bias = 0;
+    for (int32_t t0 = 0; t0 < total; t0 += T * U) {
+      bool ok[U];
+      int j[U];
+      int64_t q[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
+        ok[u] = t < total;
+        t = ok[u] ? t : total - 1;
+        if (t >= seg_end) {
+          cur = seek_segment(L.off, nvalid, cur, t);
+          seg_end = cur + 1 < nvalid ? L.off[cur + 1] : total;
+          seg_bias = L.start[cur] - (int64_t)L.off[cur];
         }
+        j[u] = cur;
+        q[u] = seg_bias + t;
       }
-      body(valid, valid ? lo : -1, w0 + lo, valid ? L.start[lo] + (int64_t)(t - L.off[lo]) : 0);
+      body(ok, j, q);
     }
     __syncthreads();
     if (threadIdx.x < nvalid) flush(threadIdx.x, w0 + threadIdx.x);
     __syncthreads();
   }
 }
+
+// Products in flight per lane in the backward expansion.
+constexpr int kBwdU = 4;
 
 // Rows whose G row fits an LDS table: one row per block.
 template <int S, int T, typename V>
@@ -102,7 +146,7 @@ __global__ __launch_bounds__(T) void k_bwd_hash(Csr A, Csr B, Csr G, const int32
       else gv[h] = gval[e];
     }
     __syncthreads();
-    block_expand_flush<T>(
+    block_expand_flush<T, kBwdU>(
         A.rp[row], A.rp[row + 1], L,
         [&](int64_t p, int64_t& st, int32_t& len) {
           const int32_t k = A.col[p];
@@ -111,16 +155,26 @@ __global__ __launch_bounds__(T) void k_bwd_hash(Csr A, Csr B, Csr G, const int32
           aval[threadIdx.x] = av[p];
           sda[threadIdx.x] = V(0);
         },
-        [&](bool valid, int j, int64_t, int64_t q) {
-          V contrib = V(0);
-          if (valid) {
-            const uint32_t slot = lds_find(keys, shift, B.col[q]);
-            const V g = slot < (uint32_t)S ? gv[slot] : V(0);
-            if (slot >= (uint32_t)S) atomicOr(err, 64);
-            unsafeAtomicAdd(&dB[q], aval[j] * g);
-            contrib = g * bv[q];
+        [&](bool (&ok)[kBwdU], int (&j)[kBwdU], int64_t (&q)[kBwdU]) {
+          int32_t bc[kBwdU];
+          V bq[kBwdU];
+#pragma unroll
+          for (int u = 0; u < kBwdU; ++u) {
+            bc[u] = B.col[q[u]];
+            bq[u] = bv[q[u]];
           }
-          if (wave_run_sum<V>(j, contrib) && valid) atomicAdd(&sda[j], contrib);
+#pragma unroll
+          for (int u = 0; u < kBwdU; ++u) {
+            V contrib = V(0);
+            if (ok[u]) {
+              const uint32_t slot = lds_find(keys, shift, bc[u]);
+              const V g = slot < (uint32_t)S ? gv[slot] : V(0);
+              if (slot >= (uint32_t)S) atomicOr(err, 64);
+              unsafeAtomicAdd(&dB[q[u]], aval[j[u]] * g);
+              contrib = g * bq[u];
+            }
+            if (wave_seg_sum<V>(ok[u] ? j[u] : -1, contrib) && ok[u]) lds_add(&sda[j[u]], contrib);
+          }
         },
         [&](int j, int64_t p) { dA[p] = sda[j]; });
   }
@@ -173,7 +227,7 @@ __global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const
     for (int64_t e = e0 + threadIdx.x; e < e1; e += T) gd[G.col[e] - c0] = gval[e];
     __syncthreads();
     const int64_t* bp = bptr + (int64_t)c * ld;
-    block_expand_flush<T>(
+    block_expand_flush<T, kBwdU>(
         A.rp[row], A.rp[row + 1], L,
         [&](int64_t p, int64_t& st, int32_t& len) {
           const int32_t k = A.col[p];
@@ -182,14 +236,27 @@ __global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const
           aval[threadIdx.x] = av[p];
           sda[threadIdx.x] = V(0);
         },
-        [&](bool valid, int j, int64_t, int64_t q) {
-          V contrib = V(0);
-          if (valid) {
-            const V g = gd[bcol[q]];
-            unsafeAtomicAdd(&dB[perm[q]], aval[j] * g);
-            contrib = g * bval[q];
+        [&](bool (&ok)[kBwdU], int (&j)[kBwdU], int64_t (&q)[kBwdU]) {
+          // every lane's loads go out before the first is used
+          uint32_t bc[kBwdU];
+          V bq[kBwdU];
+          int64_t pm[kBwdU];
+#pragma unroll
+          for (int u = 0; u < kBwdU; ++u) {
+            bc[u] = bcol[q[u]];
+            bq[u] = bval[q[u]];
+            pm[u] = perm[q[u]];
           }
-          if (wave_run_sum<V>(j, contrib) && valid) atomicAdd(&sda[j], contrib);
+#pragma unroll
+          for (int u = 0; u < kBwdU; ++u) {
+            const V g = gd[bc[u]];
+            V contrib = V(0);
+            if (ok[u]) {
+              unsafeAtomicAdd(&dB[pm[u]], aval[j[u]] * g);
+              contrib = g * bq[u];
+            }
+            if (wave_seg_sum<V>(ok[u] ? j[u] : -1, contrib) && ok[u]) lds_add(&sda[j[u]], contrib);
+          }
         },
         [&](int j, int64_t p) {
           if (sda[j] != V(0)) unsafeAtomicAdd(&dA[p], sda[j]);

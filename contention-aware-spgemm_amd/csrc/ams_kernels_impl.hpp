@@ -2707,11 +2707,23 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
             constexpr int g0 = gstart(decltype(gi_c)::value);
             constexpr int G = gsize(decltype(gi_c)::value);
             uint32_t kk[G], bw[G];
+            if constexpr (sizeof(V) == 8) {  // ds_add_f64, each add beside its bitmap read
 #pragma unroll
-            for (int j = 0; j < G; ++j) {
-              kk[j] = ok[g0 + j] ? item[g0 + j].key : (uint32_t)(W + lane_id());
-              atomicAdd(&dense[kk[j]], item[g0 + j].a * item[g0 + j].val);
-              bw[j] = bm[kk[j] >> 5];
+              for (int j = 0; j < G; ++j) {
+                kk[j] = ok[g0 + j] ? item[g0 + j].key : (uint32_t)(W + lane_id());
+                atomicAdd(&dense[kk[j]], item[g0 + j].a * item[g0 + j].val);
+                bw[j] = bm[kk[j] >> 5];
+              }
+            } else {  // the group's FP32 swaps back to back
+              V pv[G];
+#pragma unroll
+              for (int j = 0; j < G; ++j) {
+                kk[j] = ok[g0 + j] ? item[g0 + j].key : (uint32_t)(W + lane_id());
+                pv[j] = item[g0 + j].a * item[g0 + j].val;
+              }
+              lds_add_group<G>(dense, kk, pv);
+#pragma unroll
+              for (int j = 0; j < G; ++j) bw[j] = bm[kk[j] >> 5];
             }
 #pragma unroll
             for (int j = 0; j < G; ++j) {
