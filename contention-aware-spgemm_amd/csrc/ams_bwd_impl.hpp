@@ -180,15 +180,52 @@ __global__ __launch_bounds__(T) void k_bwd_hash(Csr A, Csr B, Csr G, const int32
   }
 }
 
+// dB[perm[q]] += dBb[q]: the dense bin's dB partials, kept in the order of
+// B's column-blocked copy (q), added at B's own slots (perm is a bijection,
+// so no two threads write one slot).
+template <typename V>
+__global__ __launch_bounds__(256) void k_bwd_unpermute(const int64_t* perm, const V* dBb, V* dB, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    dB[perm[q]] += dBb[q];
+}
+
+// First G entry of every (dense-bin row h, column chunk c), c = 0..R:
+// gsplit[h * (R + 1) + c] = lower bound of the chunk's first column in G's
+// row (G rows are sorted), one thread per entry, so the morsels of the
+// backward read their range instead of searching G's row.
+__global__ __launch_bounds__(256) void k_bwd_gsplit(Csr G, const int32_t* list, const int32_t* counts, int bin, int R,
+                                                    int W, Probs pr, int64_t* gsplit) {
+  int32_t start, H;
+  bin_range(counts, bin, start, H);
+  const int64_t n = (int64_t)H * (R + 1);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t h = (int32_t)(i / (R + 1));
+    const int c = (int)(i - (int64_t)h * (R + 1));
+    const int32_t row = list[start + h];
+    int64_t lo, width;
+    prob_cols(pr, row, lo, width);
+    const int64_t c0 = lo + (int64_t)c * W;
+    int64_t a = G.rp[row], b = G.rp[row + 1];
+    while (a < b) {
+      const int64_t mid = (a + b) >> 1;
+      if (G.col[mid] < c0) a = mid + 1;
+      else b = mid;
+    }
+    gsplit[i] = a;
+  }
+}
+
 // Rows whose G row is too wide for a table: persistent blocks take
 // (row, column chunk) tickets like the forward's dense bin; G's chunk is
 // scattered into a dense LDS array, products come from the column-blocked
-// B and reach B's original slot through perm.
+// B, and dB accumulates in the blocked order (dBb[q], consecutive for the
+// consecutive products of a segment) for k_bwd_unpermute.
 template <int T, typename V>
 __global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const int64_t* bptr, const uint16_t* bcol,
-                                                 const V* bval, const int64_t* perm, const int32_t* list,
+                                                 const V* bval, const int32_t* list,
                                                  const int32_t* counts, int bin, int R, int W, Probs pr,
-                                                 unsigned long long* ticket, V* dA, V* dB) {
+                                                 const int64_t* gsplit, unsigned long long* ticket, V* dA,
+                                                 V* dBb) {
   extern __shared__ __align__(16) unsigned char dyn_raw[];
   V* gd = (V*)dyn_raw;
   __shared__ ExpandLds<T> L;
@@ -209,20 +246,13 @@ __global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const
     const unsigned long long tk = s_ticket;
     if (tk >= ntask) break;
     const int c = (int)(tk / (unsigned long long)H);
-    const int32_t row = list[start + (int32_t)(tk - (unsigned long long)c * H)];
+    const int32_t h = (int32_t)(tk - (unsigned long long)c * H);
+    const int32_t row = list[start + h];
     int64_t lo, width;
     prob_cols(pr, row, lo, width);
-    const int64_t c0 = lo + (int64_t)c * W, c1 = c0 + W;
-    // G's entries of this chunk: G rows are sorted, so a binary search finds them
-    int64_t e0 = G.rp[row], e1 = G.rp[row + 1];
-    {
-      int64_t a = e0, b = e1;
-      while (a < b) { const int64_t mid = (a + b) >> 1; if (G.col[mid] < c0) a = mid + 1; else b = mid; }
-      e0 = a;
-      b = e1;
-      while (a < b) { const int64_t mid = (a + b) >> 1; if (G.col[mid] < c1) a = mid + 1; else b = mid; }
-      e1 = a;
-    }
+    const int64_t c0 = lo + (int64_t)c * W;
+    // G's entries of this chunk
+    const int64_t e0 = gsplit[(int64_t)h * (R + 1) + c], e1 = gsplit[(int64_t)h * (R + 1) + c + 1];
     if (e0 == e1) continue;
     for (int64_t e = e0 + threadIdx.x; e < e1; e += T) gd[G.col[e] - c0] = gval[e];
     __syncthreads();
@@ -240,19 +270,17 @@ __global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const
           // every lane's loads go out before the first is used
           uint32_t bc[kBwdU];
           V bq[kBwdU];
-          int64_t pm[kBwdU];
 #pragma unroll
           for (int u = 0; u < kBwdU; ++u) {
             bc[u] = bcol[q[u]];
             bq[u] = bval[q[u]];
-            pm[u] = perm[q[u]];
           }
 #pragma unroll
           for (int u = 0; u < kBwdU; ++u) {
             const V g = gd[bc[u]];
             V contrib = V(0);
             if (ok[u]) {
-              unsafeAtomicAdd(&dB[pm[u]], aval[j[u]] * g);
+              unsafeAtomicAdd(&dBb[q[u]], aval[j[u]] * g);
               contrib = g * bq[u];
             }
             if (wave_seg_sum<V>(ok[u] ? j[u] : -1, contrib) && ok[u]) lds_add(&sda[j[u]], contrib);

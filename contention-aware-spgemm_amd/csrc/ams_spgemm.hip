@@ -129,6 +129,7 @@ struct ams_ctx {
   Buf pool_rp, pool_col, pool_val, probs, prow, pbrow, scratch_a, dflag, dtask, ccnt, lrows;
   Buf hflag, hrow, hpiece, gbm;  // heavy bitmap rows: flags, slots, pieces, global bitmaps
   Buf gtab, gnum;                // global-hash tables: symbolic keys, numeric regions
+  Buf bgrad;                     // backward: dB partials of the dense bin in blocked-B order
   int64_t ghash_num_bytes = 0;   // of the last symbolic plan
   size_t gbm_zero = 0;           // bytes of gbm known to be zero
   bool bblk_ready = false;  // blocked B of the current plan already built
@@ -1014,13 +1015,26 @@ static int masked_bwd_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     HIPCHK(hipFuncSetAttribute((const void*)k_bwd_dense<kDenseT, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dyn));
     CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
+    CHK(grow(ctx, ctx->bgrad, sizeof(V) * std::max<int64_t>(B.nnz, 1)));
+    V* dBb = (V*)ctx->bgrad.p;
+    if (sizeof(V) == 8) CHK(fill_i64(ctx, (int64_t*)dBb, B.nnz, 0));
+    else CHK(fill_i32(ctx, (int32_t*)dBb, B.nnz, 0));
+    // G's chunk boundaries of every dense row (the forward's dense descriptors are not kept)
+    const int64_t nsplit = (int64_t)ctx->cnt_num[NB_DENSE] * (R + 1);
+    CHK(grow(ctx, ctx->dflag, sizeof(int64_t) * std::max<int64_t>(nsplit, 1)));
+    const int gs = (int)std::max<int64_t>(1, std::min<int64_t>((nsplit + 255) / 256, (int64_t)ctx->num_cu * 16));
+    hipLaunchKernelGGL(k_bwd_gsplit, dim3(gs), dim3(256), 0, ctx->stream, G, (const int32_t*)ctx->lnum.p,
+                       (const int32_t*)c->cnt_num, (int)NB_DENSE, R, W, pr, (int64_t*)ctx->dflag.p);
     int occ;
     occupancy(k_bwd_dense<kDenseT, V>, kDenseT, dyn, &occ);
     const int g = grid_for(ctx, occ, (int64_t)ctx->cnt_num[NB_DENSE] * R, true);
     hipLaunchKernelGGL((k_bwd_dense<kDenseT, V>), dim3(g), dim3(kDenseT), dyn, ctx->stream, A, G, B.m,
                        (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
-                       (const int64_t*)ctx->bperm.p, (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num,
-                       (int)NB_DENSE, R, W, pr, &c->ticket, dA, dB);
+                       (const int32_t*)ctx->lnum.p, (const int32_t*)c->cnt_num,
+                       (int)NB_DENSE, R, W, pr, (const int64_t*)ctx->dflag.p, &c->ticket, dA, dBb);
+    const int gu = (int)std::max<int64_t>(1, std::min<int64_t>((B.nnz + 255) / 256, (int64_t)ctx->num_cu * 16));
+    hipLaunchKernelGGL((k_bwd_unpermute<V>), dim3(gu), dim3(256), 0, ctx->stream, (const int64_t*)ctx->bperm.p,
+                       (const V*)dBb, dB, B.nnz);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(&ctx->h_ctrl->err, &c->err, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1103,7 +1117,7 @@ int ams_ctx_destroy(ams_ctx_t* ctx) {
                  &ctx->bptr, &ctx->bcol, &ctx->bval, &ctx->bperm, &ctx->tsum, &ctx->pool_rp, &ctx->pool_col,
                  &ctx->pool_val, &ctx->probs, &ctx->scratch_a, &ctx->dflag, &ctx->dtask, &ctx->ccnt,
                  &ctx->prow, &ctx->pbrow, &ctx->lrows, &ctx->hflag, &ctx->hrow, &ctx->hpiece, &ctx->gbm,
-                 &ctx->gtab, &ctx->gnum};
+                 &ctx->gtab, &ctx->gnum, &ctx->bgrad};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : ctx->recs) {
