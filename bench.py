@@ -424,6 +424,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched, fwd+bwd and other-workload sections")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuned rerun of the workload")
     ap.add_argument("--param", action="append", default=[], help="routing parameter key=value")
+    ap.add_argument("--only", choices=["fwd_bwd", "batched"],
+                    help="run only this section (for profiling) and print it as one JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -447,6 +449,10 @@ def main():
     from ams_hip.device import Context, DeviceCsr
 
     torch.cuda.set_device(0)
+    if args.only:
+        sec = bench_fwd_bwd(18, 16, args.steps) if args.only == "fwd_bwd" else bench_batched(512, SEED, args.steps)
+        print(json.dumps({args.only: sec}), flush=True)
+        return
     t0 = time.perf_counter()
     a, b = make_inputs(args.workload)
     log(f"inputs {args.workload}: A {a.num_row}x{a.num_col} nnz {a.num_nnz}, generated in "

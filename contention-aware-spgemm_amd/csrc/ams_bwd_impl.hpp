@@ -239,12 +239,17 @@ __global__ __launch_bounds__(T) void k_bwd_dense(Csr A, Csr G, int64_t mB, const
   const V* av = (const V*)A.val;
   const V* gval = (const V*)G.val;
   const int64_t ld = mB + 1;
+  // thread 0 claims the next morsel while the block works on the current
+  // one, so the claim's latency is not waited for
+  unsigned long long nxt = 0;
+  if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
   for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1ull);
+    __syncthreads();  // every thread has read the last ticket
+    if (threadIdx.x == 0) s_ticket = nxt;
     __syncthreads();
     const unsigned long long tk = s_ticket;
     if (tk >= ntask) break;
+    if (threadIdx.x == 0) nxt = atomicAdd(ticket, 1ull);
     const int c = (int)(tk / (unsigned long long)H);
     const int32_t h = (int32_t)(tk - (unsigned long long)c * H);
     const int32_t row = list[start + h];
