@@ -175,17 +175,20 @@ def test_esc_long_a_rows(gpu_ctx, route, b_density):
     assert_matches_oracle(c, a, b, exact=False)
 
 
-def test_esc_wide_columns(gpu_ctx):
+@pytest.mark.parametrize("density", [0.05, 0.2, 0.4])
+def test_esc_wide_columns(gpu_ctx, density):
     """ESC rows over 2^26 + 7 columns: (column, rank) no longer packs into 32
-    bits for any bin, so the 64-bit key sort runs (random FP64 values)."""
+    bits for any bin, so the 64-bit key sort runs (random FP64 values).  Rows
+    of about 300, 1200 and 2400 products: the 1536- and 3072-product bins then
+    run the 64-bit kernel with the 2048 and 4096 geometry."""
     rng = np.random.default_rng(17)
     n = (1 << 26) + 7
-    a = random_csr(200, 300, 0.05, seed=18)
+    a = random_csr(200, 300, density, seed=18)
     rows = np.repeat(np.arange(300), 20)
     cols = np.sort(rng.choice(n, size=(300, 20), replace=False if n > 20 else True), axis=1).reshape(-1)
     rp = np.arange(0, 301 * 20, 20, dtype=np.int64)
     b = Csr(300, n, rp, cols.astype(np.int32), rng.uniform(-1, 1, len(cols)))
-    print("[parity] esc over 2^26+7 columns", flush=True)
+    print(f"[parity] esc over 2^26+7 columns, A density {density}", flush=True)
     c = run(gpu_ctx, a, b, dict(mode=2, esc_max=4096))
     assert c.sorted_rows()
     assert_matches_oracle(c, a, b, exact=False)
@@ -302,6 +305,8 @@ def test_fp32_chunk_width(gpu_ctx):
 ESC_CASES = [("rmat_s8_ef4", lambda: gen.rmat_symmetric(8, 4, seed=101, values="uniform")),
              ("rmat_s10_ef8", lambda: gen.rmat_symmetric(10, 8, seed=102, values="uniform")),
              ("er_s11_ef8", lambda: gen.rmat_symmetric(11, 8, gen.ER, seed=103, values="uniform")),
+             # rows of about 2400 products: the 1536, 2048 and 3072 bins
+             ("er_s11_ef24", lambda: gen.rmat_symmetric(11, 24, gen.ER, seed=104, values="uniform")),
              ("rect_unsorted", None)]
 
 
