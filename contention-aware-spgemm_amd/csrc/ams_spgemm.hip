@@ -329,6 +329,11 @@ struct Phase {
   }
 };
 
+// Blocks a CU the backward's dense chunks are narrowed to fit.
+#ifndef AMS_BWD_MIN_BLOCKS
+#define AMS_BWD_MIN_BLOCKS 2
+#endif
+
 // Geometry of the dense column chunks and the symbolic bitmap passes.
 // dense_width is the FP64 chunk width; FP32 chunks are 7/4 as wide (the
 // 4-byte accumulators leave the LDS room, and fewer, larger morsels pay
@@ -1009,6 +1014,18 @@ static int masked_bwd_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
     const Probs pr = single_probs(b);
     int W, R, Wb;
     dense_geometry(ctx, sizeof(V) == 8 ? AMS_F64 : AMS_F32, pr.max_width, W, R, Wb);
+    // The backward builds its own blocked copy of B, so its chunks may be
+    // narrower than the forward's: at least two blocks a CU (the FP64
+    // forward chunk, 67.6 KB of G values next to this kernel's 14 KB of
+    // static LDS, left room for one: 164 ms against 73 ms in FP32 on G500
+    // s18 ef16).
+    for (int occ = 0;;) {
+      occupancy(k_bwd_dense<kDenseT, V>, kDenseT, (size_t)W * sizeof(V), &occ);
+      if (occ >= AMS_BWD_MIN_BLOCKS || W <= 1024) break;
+      W -= 256;
+    }
+    R = (int)((pr.max_width + W - 1) / W);
+    if (R < 1) R = 1;
     CHK(build_bblk<V>(ctx, B, pr, W, R, true));
     ctx->bblk_ready = false;
     const size_t dyn = (size_t)W * sizeof(V);
