@@ -84,8 +84,16 @@ constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 #ifndef AMS_HASH_TMAX
 #define AMS_HASH_TMAX 1024
 #endif
+#ifndef AMS_HASH_TDIV
+#define AMS_HASH_TDIV 8
+#endif
 template <int S>
-constexpr int hash_threads() { return S / 8 < 64 ? 64 : (S / 8 > AMS_HASH_TMAX ? AMS_HASH_TMAX : S / 8); }
+constexpr int hash_threads() {
+  return S / AMS_HASH_TDIV < 64 ? 64 : (S / AMS_HASH_TDIV > AMS_HASH_TMAX ? AMS_HASH_TMAX : S / AMS_HASH_TDIV);
+}
+// Threads of an ESC block: eight keys a thread (E / 8), at least one wave.
+template <int E>
+constexpr int esc_threads() { return E / 8 < 64 ? 64 : E / 8; }
 
 // The symbolic hash keeps keys only (4S bytes of LDS): half the threads of
 // the numeric table's block, so twice the rows per CU are in flight at the
@@ -791,7 +799,7 @@ static void launch_numc_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, const i
 template <int E, typename V, bool K32>
 static void launch_numc_esc_k(ams_ctx_t* ctx, const Csr& A, const Csr& B, const int64_t* rp_c, int32_t* col_c,
                               V* val_c, int64_t cap, int bin, int64_t cnt) {
-  constexpr int T = hash_threads<E>();
+  constexpr int T = esc_threads<E>();
   if constexpr (K32) {
     {
       // register bitonic sort (esc_sort 0), LSD radix sort by the column bits
