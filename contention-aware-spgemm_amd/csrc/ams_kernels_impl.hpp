@@ -2438,6 +2438,11 @@ constexpr unsigned long long kGrab = 8;
 // take one set bit per lane; denser groups take a word per half-wave, one
 // bit per lane (G500 s19 ef32: 336.6 -> 319.9 ms; 4: 328, 16: 326 ms).
 constexpr int kSparseEmitBits = 8;
+// Staging entries a wave for the sparse emission (a sparse group has fewer
+// than kSparseEmitBits * 64 set bits); T of them fill the segment
+// descriptors' LDS (T * sizeof(SegBias) bytes).
+constexpr int kSparseStage = kSparseEmitBits * 64;
+static_assert(kSparseStage * 2 <= kWave * 16, "a wave's staging row fits its share of the descriptors");
 
 // A morsel descriptor held as one dword per lane (lanes 0..7 of every
 // wave): it stays a vector load, so waits on LDS results never also wait
@@ -2796,33 +2801,28 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
       const int32_t winc = wave_incl_scan(wcnt);
       const int32_t gtot = wave_last(winc);  // set bits of the group
       if (gtot < kSparseEmitBits * __popcll(nz)) {
-        // sparse words: lane l emits the group's set bit o0 + l.  Its word
-        // is the last lane j with wex_j <= o (empty words share the next
-        // word's wex, so the non-empty one wins), found by binary lifting
-        // over the lanes; its bit is the k-th set bit of that word, found
-        // by halving popcounts.  Every step stores 64 consecutive entries.
-        const int32_t wex = winc - wcnt;
+        // sparse words (fewer than 512 set bits in the group): every lane
+        // writes the (word, bit) codes of its word's set bits at their
+        // ranks into the wave's staging row, then lane l emits the group's
+        // set bit o0 + l from it.  The staging rows are the segment
+        // descriptors' LDS, unused from the morsel's last pass until the
+        // next morsel's first window scan, which every wave reaches only
+        // after its emission.  Every step stores 64 consecutive entries.
+        uint16_t* stg = (uint16_t*)sb + wave_id() * kSparseStage;
+        {
+          uint32_t y = word;
+          int32_t r = winc - wcnt;
+          while (y) {
+            stg[r++] = (uint16_t)((lane_id() << 5) | __builtin_ctz(y));
+            y &= y - 1u;
+          }
+        }
+        wave_lds_sync();
         for (int32_t o0 = 0; o0 < gtot; o0 += kWave) {
           const int32_t o = o0 + lane_id();
-          int j = 0;
-#pragma unroll
-          for (int s = kWave / 2; s >= 1; s >>= 1) {
-            const int32_t v = __shfl(wex, j + s, kWave);
-            j = v <= o ? j + s : j;
-          }
-          uint32_t y = (uint32_t)__shfl((int)word, j, kWave);
-          int32_t k = o - __shfl(wex, j, kWave);
-          int pos = 0;
-#pragma unroll
-          for (int h = 16; h >= 1; h >>= 1) {
-            const int32_t c = __popc(y & ((1u << h) - 1u));
-            const bool up = k >= c;
-            k = up ? k - c : k;
-            y = up ? y >> h : y;
-            pos = up ? pos + h : pos;
-          }
           const bool valid = o < gtot;
-          const int u = ((g + j) << 5) + pos;
+          const uint32_t e = valid ? stg[o] : 0u;
+          const int u = (g << 5) + (int)e;
           const int32_t r = base + o;
           V v = V(0);
           if (valid) v = lds_take(&dense[u]);
@@ -2831,6 +2831,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
             valp[r] = v;
           }
         }
+        wave_lds_sync();  // the staging row is read before the next group writes it
         base += gtot;
         continue;
       }
