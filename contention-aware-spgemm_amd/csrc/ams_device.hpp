@@ -123,6 +123,53 @@ __device__ __forceinline__ I block_excl_scan_once(I x, I* s_wave, I& total) {
   return base + incl - x;
 }
 
+// Inclusive prefix sum over lanes 0 .. T/64 - 1 (one 16-lane DPP row):
+// combines one value per wave of a T-thread block.
+template <int T>
+__device__ __forceinline__ uint32_t wave_row_scan(uint32_t v) {
+  static_assert(T % kWave == 0 && T / kWave <= 16, "one lane per wave, inside one DPP row");
+  if constexpr (T / kWave > 1) v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+  if constexpr (T / kWave > 2) v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+  if constexpr (T / kWave > 4) v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  if constexpr (T / kWave > 8) v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+  return v;
+}
+
+// Exclusive prefix (at the calling wave) and total of the per-wave values
+// s[0 .. T/64) that every wave wrote before a barrier: lane w reads wave w's
+// value, so the VALU work does not grow with the wave count.
+template <int T>
+__device__ __forceinline__ uint32_t block_wave_prefix(const uint32_t* s, uint32_t& total) {
+  const uint32_t v = lane_id() < T / kWave ? s[lane_id()] : 0u;
+  const uint32_t incl = wave_row_scan<T>(v);
+  total = (uint32_t)__builtin_amdgcn_readlane((int)incl, T / kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(wave_id());
+  return (uint32_t)__builtin_amdgcn_readlane((int)(incl - v), wv);
+}
+
+// Segment scan of a window of T segment lengths (one per thread, each >= 0):
+// off = products before this segment, ci = non-empty segments before it
+// (its compact index), total = products of the window (must fit 31 bits).
+// A 32-bit DPP scan and a ballot per wave, then one barrier and a per-lane
+// combine of the waves' (products, non-empty) pairs in s_wave[T/64].  No
+// closing barrier: s_wave must not be written again before every thread has
+// passed another barrier.
+template <int T>
+__device__ __forceinline__ void block_seg_scan(int32_t len, uint2* s_wave, int32_t& off, int32_t& ci,
+                                               int32_t& total) {
+  const uint32_t incl = (uint32_t)wave_incl_scan(len);
+  const unsigned long long ne = __ballot(len > 0);
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
+  if (lane_id() == kWave - 1) s_wave[wave_id()] = make_uint2(incl, (uint32_t)__popcll(ne));
+  __syncthreads();
+  const uint2 v = lane_id() < T / kWave ? s_wave[lane_id()] : make_uint2(0u, 0u);
+  const uint32_t ia = wave_row_scan<T>(v.x), ib = wave_row_scan<T>(v.y);
+  const int wv = __builtin_amdgcn_readfirstlane(wave_id());
+  total = __builtin_amdgcn_readlane((int)ia, T / kWave - 1);
+  off = __builtin_amdgcn_readlane((int)(ia - v.x), wv) + (int32_t)incl - len;
+  ci = __builtin_amdgcn_readlane((int)(ib - v.y), wv) + (int32_t)below;
+}
+
 template <int T, typename I>
 __device__ __forceinline__ I block_sum(I x, I* s_wave) {
   I total;

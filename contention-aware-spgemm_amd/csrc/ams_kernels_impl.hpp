@@ -2520,8 +2520,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   __shared__ uint32_t pbits[3][kPassWords];
   __shared__ uint2 wpword[T / kWave][2 * U];  // per wave: (start bits, set bits before) of its pass words
   __shared__ int32_t cover[3];
-  __shared__ int64_t s_w[2][T / kWave];
-  __shared__ int32_t s_rank[T / kWave];
+  __shared__ uint2 s_w[2][T / kWave];
+  __shared__ uint32_t s_rank[T / kWave];
   __shared__ unsigned long long s_ring[4];
   if (rp_c[A.m] > cap) return;
   const int words = (W + kWave) >> 5;
@@ -2580,13 +2580,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     const uint32_t nndv = task_load(tasks, t_nn, ntask);  // descriptor two morsels ahead
     const int64_t p1 = d.p0 + d.deg;
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
-      // scan of (non-empty count << 32 | length): offsets and compact index
-      int64_t tot64;
-      const int64_t ex = block_excl_scan_once<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len,
-                                                          s_w[wc++ & 1], tot64);
-      const int32_t off = (int32_t)(ex & 0xffffffffll);
-      const int32_t ci = (int32_t)(ex >> 32);
-      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
+      // scan of the segment lengths: offsets and compact index
+      int32_t off, ci, total;
+      block_seg_scan<T>(len, s_w[wc++ & 1], off, ci, total);
       // prefetch the A entries of the next window: this row's next one, or
       // the next morsel's first; their B bounds follow after the passes
       const bool last = w0 + T >= p1;
@@ -2774,16 +2770,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     int32_t pc = __popc(word0);
     for (int w = wv0 + kWave + lane_id(); w < wv1; w += kWave) pc += __popc(bm[w]);
     pc = wave_sum(pc);
-    if (lane_id() == 0) s_rank[wave_id()] = pc;
+    if (lane_id() == 0) s_rank[wave_id()] = (uint32_t)pc;
     __syncthreads();
-    int32_t base = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < T / kWave; ++w) {
-      const int32_t v = s_rank[w];
-      base += w < wave_id() ? v : 0;
-      tot += v;
-    }
-    if (threadIdx.x == 0 && tot != d.cnt) atomicOr(err, 4);
+    uint32_t utot;
+    int32_t base = (int32_t)block_wave_prefix<T>(s_rank, utot);
+    if (threadIdx.x == 0 && (int32_t)utot != d.cnt) atomicOr(err, 4);
     const int half = lane_id() >> 5, bit = lane_id() & 31;
     const unsigned long long below = (1ull << lane_id()) - 1;
     for (int g = wv0; g < wv1; g += kWave) {
