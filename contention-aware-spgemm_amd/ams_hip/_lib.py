@@ -27,7 +27,7 @@ class AmsParams(C.Structure):
                 ("acc_thres", C.c_double), ("dense_width", C.c_int32), ("bitmap_bits", C.c_int32),
                 ("bitmap_ratio", C.c_double), ("dense_density", C.c_double), ("mode", C.c_int32),
                 ("wgs_per_cu", C.c_int32), ("esc_sort", C.c_int32),
-                ("sym_blocked", C.c_int32), ("ghash_ratio", C.c_double)]
+                ("sym_blocked", C.c_int32), ("ghash_ratio", C.c_double), ("light_max", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -50,7 +50,7 @@ class AmsStats(C.Structure):
 
 
 PHASES = ["flops", "classify", "symb_hash", "symb_bitmap", "scan", "bblk", "numc_dense", "numc_hash",
-          "numc_esc", "symb_ghash", "numc_ghash"]
+          "numc_esc", "symb_ghash", "numc_ghash", "numc_light"]
 
 
 # name -> (restype, argtypes); every symbol declared in include/ams_hip.h.

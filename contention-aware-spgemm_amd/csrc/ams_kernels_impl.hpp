@@ -2383,10 +2383,16 @@ struct DenseTask {
 };
 static_assert(sizeof(DenseTask) == 40, "descriptor dwords");
 
-// flag[c*H + h] = 1 when morsel (h, c) is routed dense and not empty.
+// Light morsels: at most this many A nonzeros (a wave reads them 64 at a
+// time, so a long row stays with the block kernel).
+constexpr int kLightDeg = 256;
+
+// flag[c*H + h] for morsel (h, c) routed dense and not empty: 1 for the
+// block kernel, 1 << 32 for the wave kernel of light morsels (at most
+// light_max C entries and kLightDeg A nonzeros; light_max 0: none).
 __global__ __launch_bounds__(256) void k_dense_flags(const int32_t* list, const int32_t* counts, int bin,
                                                      const uint8_t* nbin, const int32_t* coff, int R,
-                                                     int64_t* flag) {
+                                                     const int64_t* arp, int light_max, int64_t* flag) {
   int32_t start, H;
   bin_range(counts, bin, start, H);
   const int64_t n = (int64_t)H * R;
@@ -2395,13 +2401,19 @@ __global__ __launch_bounds__(256) void k_dense_flags(const int32_t* list, const 
     const int32_t h = (int32_t)(i - (int64_t)c * H);
     const int32_t row = list[start + h];
     const int32_t* crow = coff + (int64_t)h * (R + 1);
-    flag[i] = (nbin[row] == NB_DENSE && crow[c + 1] > crow[c]) ? 1 : 0;
+    const int32_t cnt = crow[c + 1] - crow[c];
+    int64_t f = 0;
+    if (nbin[row] == NB_DENSE && cnt > 0)
+      f = (cnt <= light_max && arp[row + 1] - arp[row] <= kLightDeg) ? (1ll << 32) : 1;
+    flag[i] = f;
   }
 }
 
 // Writes the descriptors of the flagged morsels at their scanned positions
-// (pos = exclusive scan of the flags, n_all + 1 entries), keeping the
-// chunk-major order, and the morsel count into *ntask.
+// (pos = exclusive scan of the flags, n_all + 1 entries; block-kernel
+// morsels counted in the low 32 bits, light ones in the high 32), keeping
+// the chunk-major order: the block kernel's morsels first, then the light
+// ones.  Their counts go to ntask[0] and ntask[1].
 __global__ __launch_bounds__(256) void k_dense_fill(Csr A, Probs pr, int W, const int32_t* list,
                                                     const int32_t* counts, int bin, const int32_t* coff, int R,
                                                     const int64_t* rp_c, const int64_t* pos, DenseTask* tasks,
@@ -2409,9 +2421,11 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, Probs pr, int W, cons
   int32_t start, H;
   bin_range(counts, bin, start, H);
   const int64_t n = (int64_t)H * R;
+  const int64_t nheavy = pos[n] & 0xffffffffll;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = pos[i];
-    if (pos[i + 1] == p) continue;  // not flagged
+    const int64_t pv = pos[i], pn = pos[i + 1];
+    if (pn == pv) continue;  // not flagged
+    const int64_t p = (pn & 0xffffffffll) != (pv & 0xffffffffll) ? (pv & 0xffffffffll) : nheavy + (pv >> 32);
     const int c = (int)(i / H);
     const int32_t h = (int32_t)(i - (int64_t)c * H);
     const int32_t row = list[start + h];
@@ -2430,7 +2444,10 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, Probs pr, int W, cons
     d.cwidth = (int32_t)(cw < W ? cw : W);
     tasks[p] = d;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *ntask = pos[n];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ntask[0] = nheavy;
+    ntask[1] = pos[n] >> 32;
+  }
 }
 
 constexpr unsigned long long kGrab = 8;
@@ -2855,6 +2872,219 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     ndv = nndv;
     t_cur = t_nxt;
     t_nxt = t_nn;
+  }
+}
+
+// Light morsels, one wave each.  A (row, chunk) morsel of few C entries
+// costs the block kernel about as much as its fixed steps (window scan, pass
+// set-up, three barriers, ranking): most of a G500 s19 ef32 row's 62 chunks
+// hold a few hundred products.  Here a wave takes such a morsel alone, with
+// no block barrier:
+//   * windows of 64 A nonzeros: B segment bounds of the chunk, a wave scan
+//     of their lengths, (bias, a) of the non-empty ones in a per-wave table;
+//   * products in rounds of 64*kLightU positions: the segments starting in
+//     the round set bits in per-wave mark words, so position x belongs to
+//     segment (segments begun before the round) + (marks <= x) - 1;
+//   * each product a*b goes into a per-wave LDS hash table of kLightS slots
+//     (linear probing, claim by compare-and-swap; FP64 ds_add_f64, FP32 the
+//     compare-and-swap add) and its column into a chunk bitmap;
+//   * output: prefix popcounts of the bitmap words give each present
+//     column its rank, so every slot stores its entry at rank order (sorted
+//     C without a sort); slots, bitmap and marks are cleared as they are
+//     read.
+// The next morsel's descriptor and first A entries are loaded while the
+// current one runs.  Per-wave dynamic LDS (light_wave_lds):
+//   uint32 keys[kLightS] | V vals[kLightS] | uint32 bm[BW rounded to 4] |
+//   1 KB: SegBias<V> seg[64] (products), uint16 prefix[BW] (output) | u64 marks[kLightU]
+constexpr int kLightT = 256;
+constexpr int kLightS = 512;
+constexpr int kLightLg = 9;
+constexpr int kLightMaxCnt = (kLightS * 2) / 3;
+constexpr int kLightMaxWords = 512;  // chunks of up to 16384 columns
+constexpr int kLightU = 4;
+static_assert((1 << kLightLg) == kLightS, "table size");
+static_assert(kLightMaxWords * 2 <= 64 * 16, "word prefixes fit the segment table's bytes");
+
+__host__ __device__ constexpr size_t light_wave_lds(int W, int vbytes) {
+  return (size_t)kLightS * 4 + (size_t)kLightS * vbytes + (size_t)((((W + 31) >> 5) + 3) & ~3) * 4 + 1024 +
+         (size_t)kLightU * 8;
+}
+__host__ __device__ constexpr size_t light_block_lds(int W, int vbytes) {
+  return light_wave_lds(W, vbytes) * (kLightT / kWave);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const int64_t* bptr,
+                                                        const uint16_t* bcol, const V* bval,
+                                                        const DenseTask* tasks, const int64_t* __restrict__ d_ntask,
+                                                        int W, const int64_t* rp_c, int32_t* col_c, V* val_c,
+                                                        int64_t cap, int32_t* err) {
+  extern __shared__ __align__(16) unsigned char dyn_raw[];
+  if (rp_c[A.m] > cap) return;
+  const int BW = (W + 31) >> 5;
+  unsigned char* mine = dyn_raw + light_wave_lds(W, (int)sizeof(V)) * wave_id();
+  uint32_t* keys = (uint32_t*)mine;
+  V* vals = (V*)(keys + kLightS);
+  uint32_t* bm = (uint32_t*)(vals + kLightS);
+  unsigned char* tail = (unsigned char*)(bm + ((BW + 3) & ~3));
+  SegBias<V>* seg = (SegBias<V>*)tail;
+  uint16_t* pre = (uint16_t*)tail;
+  unsigned long long* marks = (unsigned long long*)(tail + 1024);
+  const int lane = lane_id();
+  for (int i = lane; i < kLightS; i += kWave) {
+    keys[i] = 0xffffffffu;
+    vals[i] = V(0);
+  }
+  for (int i = lane; i < BW; i += kWave) bm[i] = 0u;
+  if (lane < kLightU) marks[lane] = 0ull;
+  wave_lds_sync();
+  const int64_t nheavy = d_ntask[0], nlight = d_ntask[1];
+  const unsigned long long nend = (unsigned long long)(nheavy + nlight);
+  const int64_t ld = mB + 1;
+  const V* av = (const V*)A.val;
+  const int64_t nw = (int64_t)gridDim.x * (kLightT / kWave);
+  int64_t t = (int64_t)blockIdx.x * (kLightT / kWave) + wave_id();
+  if (t >= nlight) return;
+  DenseTask d = task_get(task_load(tasks, (unsigned long long)(nheavy + t), nend));
+  // this lane's A entry of the current window
+  bool has = lane < d.deg;
+  int32_t k = 0;
+  V a = V(0);
+  if (has) {
+    k = A.col[d.p0 + lane];
+    a = av[d.p0 + lane];
+  }
+  for (; t < nlight; t += nw) {
+    const int64_t tn = t + nw;
+    const uint32_t ndv = task_load(tasks, (unsigned long long)(nheavy + tn), nend);
+    const int64_t* bp = bptr + (int64_t)d.c * ld;
+    const int64_t p1 = d.p0 + d.deg;
+    DenseTask nd = d;
+    for (int64_t w0 = d.p0;; w0 += kWave) {
+      int64_t st = 0;
+      int32_t len = 0;
+      if (has) {
+        st = bp[k];
+        len = (int32_t)(bp[k + 1] - st);
+      }
+      const V aw = a;
+      // the next window's A entries: this morsel's, or the next one's first
+      const bool more = w0 + kWave < p1;
+      int64_t np = 0;
+      int32_t nn = 0;
+      if (more) {
+        np = w0 + kWave;
+        nn = (int32_t)((p1 - np) < kWave ? (p1 - np) : kWave);
+      } else if (tn < nlight) {
+        nd = task_get(ndv);
+        np = nd.p0;
+        nn = nd.deg < kWave ? nd.deg : kWave;
+      }
+      has = lane < nn;
+      if (has) {
+        k = A.col[np + lane];
+        a = av[np + lane];
+      }
+      const int32_t incl = wave_incl_scan(len);
+      const int32_t total = wave_last(incl);
+      const int32_t off = incl - len;
+      const unsigned long long ne = __ballot(len > 0);
+      const uint32_t ci =
+          __builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
+      if (len > 0) seg[ci] = SegBias<V>{st - off, aw};
+      for (int32_t t0 = 0; t0 < total; t0 += kWave * kLightU) {
+        if (len > 0 && off >= t0 && off < t0 + kWave * kLightU) {
+          const int32_t x = off - t0;
+          atomicOr(&marks[x >> 6], 1ull << (x & 63));
+        }
+        // non-empty segments begun before the round
+        const int32_t cov = (int32_t)__popcll(__ballot(len > 0 && off < t0));
+        wave_lds_sync();
+        unsigned long long mk[kLightU];
+#pragma unroll
+        for (int u = 0; u < kLightU; ++u) mk[u] = marks[u];
+        wave_lds_sync();
+        if (lane < kLightU) marks[lane] = 0ull;
+        const unsigned long long le = ~0ull >> (63 - lane);  // lanes 0..lane
+        uint32_t key[kLightU];
+        V pv[kLightU];
+        bool ok[kLightU];
+        {
+          SegBias<V> sg[kLightU];
+          int32_t cum = cov;
+#pragma unroll
+          for (int u = 0; u < kLightU; ++u) {
+            const int32_t x = t0 + u * kWave + lane;
+            ok[u] = x < total;
+            int32_t c = cum + (int32_t)__popcll(mk[u] & le) - 1;
+            c = ok[u] ? c : 0;
+            cum += (int32_t)__popcll(mk[u]);
+            sg[u] = seg[c];
+          }
+#pragma unroll
+          for (int u = 0; u < kLightU; ++u) {
+            const int32_t x = t0 + u * kWave + lane;
+            const int64_t q = ok[u] ? sg[u].bias + (int64_t)x : 0;
+            key[u] = bcol[q];
+            pv[u] = bval[q];
+          }
+#pragma unroll
+          for (int u = 0; u < kLightU; ++u) pv[u] = sg[u].a * pv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kLightU; ++u) {
+          if (!ok[u]) continue;
+          uint32_t h = (key[u] * kHashMul) >> (32 - kLightLg);
+          bool done = false;
+          for (int probe = 0; probe < kLightS; ++probe) {
+            const uint32_t old = atomicCAS(&keys[h], 0xffffffffu, key[u]);
+            if (old == 0xffffffffu) atomicOr(&bm[key[u] >> 5], 1u << (key[u] & 31));
+            if (old == 0xffffffffu || old == key[u]) {
+              lds_add(&vals[h], pv[u]);
+              done = true;
+              break;
+            }
+            h = (h + 1) & (kLightS - 1);
+          }
+          if (!done) atomicOr(err, 0x10);
+        }
+      }
+      wave_lds_sync();  // the segment table is read before the next window writes it
+      if (!more) break;
+    }
+    // ranks: prefix popcounts of the chunk bitmap's words (over the segment
+    // table, which the products no longer read)
+    int32_t run = 0;
+    for (int w0 = 0; w0 < BW; w0 += kWave) {
+      const int w = w0 + lane;
+      const uint32_t x = w < BW ? bm[w] : 0u;
+      const int32_t pc = __popc(x);
+      const int32_t inc = wave_incl_scan(pc);
+      if (w < BW) pre[w] = (uint16_t)(run + inc - pc);
+      run += wave_last(inc);
+    }
+    if (lane == 0 && run != d.cnt) atomicOr(err, 4);
+    wave_lds_sync();
+    int32_t* colp = col_c + d.off;
+    V* valp = val_c + d.off;
+    for (int i = lane; i < kLightS; i += kWave) {
+      const uint32_t key = keys[i];
+      if (key != 0xffffffffu) {
+        const uint32_t wd = bm[key >> 5];
+        const int32_t r = (int32_t)pre[key >> 5] + __popc(wd & ((1u << (key & 31)) - 1u));
+        const V v = vals[i];
+        if (r < d.cnt) {
+          colp[r] = d.colbase + (int32_t)key;
+          valp[r] = v;
+        }
+        keys[i] = 0xffffffffu;
+        vals[i] = V(0);
+      }
+    }
+    wave_lds_sync();  // every rank is read before the bitmap is cleared
+    for (int i = lane; i < BW; i += kWave) bm[i] = 0u;
+    wave_lds_sync();
+    d = nd;
   }
 }
 

@@ -42,7 +42,7 @@ struct Ctrl {
   int32_t pad;
   unsigned long long flops;
   unsigned long long ticket;
-  int64_t ntask;
+  int64_t ntask[2];  // morsels of the dense block kernel, light morsels
   int32_t nlong;  // long B rows listed for the wave-per-row blocked copy
   int32_t pad1;
   unsigned long long sums[24];     // route sums [0, 18), checksums of C [20, 23)
@@ -52,7 +52,6 @@ struct Ctrl {
   unsigned long long ghash_sym_bytes;  // global-hash tables of the symbolic route
   unsigned long long ghash_num_bytes;  // global-hash regions of the numeric route
   unsigned long long ghash_cursor;     // next free table slot / byte
-  int64_t pad2;
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl");
 
@@ -861,7 +860,32 @@ static int launch_numc_dense(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, 
   const int g = grid_for(ctx, occ, nall, ctx->plan_counts);
   hipLaunchKernelGGL((k_numc_dense<T, V>), dim3(g), dim3(T), dyn, ctx->stream, A, B.m, (const int64_t*)ctx->bptr.p,
                      (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p, (const DenseTask*)ctx->dtask.p,
-                     (const int64_t*)&c->ntask, W, rp_c, col_c, val_c, cap, pr, &c->ticket, &c->err);
+                     (const int64_t*)&c->ntask[0], W, rp_c, col_c, val_c, cap, pr, &c->ticket, &c->err);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+// Light morsels go to the wave kernel when light_max > 0 and a wave's chunk
+// bitmap fits its share of the LDS.
+static bool light_on(const ams_ctx_t* ctx, int W, int vbytes) {
+  return ctx->prm.light_max > 0 && (W + 31) / 32 <= kLightMaxWords &&
+         light_block_lds(W, vbytes) <= ctx->lds_max;
+}
+
+template <typename V>
+static int launch_numc_light(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, const int64_t* rp_c,
+                             int32_t* col_c, V* val_c, int64_t cap) {
+  Ctrl* c = dctrl(ctx);
+  const size_t dyn = light_block_lds(W, sizeof(V));
+  HIPCHK(hipFuncSetAttribute((const void*)k_numc_light<V>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+  int occ;
+  occupancy(k_numc_light<V>, kLightT, dyn, &occ);
+  // persistent waves: each walks the light morsels with the grid's stride
+  const int g = std::max(1, occ) * ctx->num_cu;
+  hipLaunchKernelGGL((k_numc_light<V>), dim3(g), dim3(kLightT), dyn, ctx->stream, A, B.m,
+                     (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
+                     (const DenseTask*)ctx->dtask.p, (const int64_t*)c->ntask, W, rp_c, col_c, val_c, cap,
+                     &c->err);
   HIPCHK(hipGetLastError());
   return AMS_OK;
 }
@@ -892,11 +916,12 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
       const int gf = (int)std::max<int64_t>(1, std::min<int64_t>((nall + 255) / 256, (int64_t)ctx->num_cu * 16));
       hipLaunchKernelGGL(k_dense_flags, dim3(gf), dim3(256), 0, ctx->stream, (const int32_t*)ctx->lsym.p,
                          (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const uint8_t*)ctx->nbin.p,
-                         (const int32_t*)ctx->coff.p, R, flag);
+                         (const int32_t*)ctx->coff.p, R, A.rp, light_on(ctx, W, sizeof(V)) ? ctx->prm.light_max : 0,
+                         flag);
       CHK(scan_i64(ctx, flag, flag, nall + 1));
       hipLaunchKernelGGL(k_dense_fill, dim3(gf), dim3(256), 0, ctx->stream, A, pr, W, (const int32_t*)ctx->lsym.p,
                          (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const int32_t*)ctx->coff.p, R, rp_c,
-                         (const int64_t*)flag, (DenseTask*)ctx->dtask.p, &c->ntask);
+                         (const int64_t*)flag, (DenseTask*)ctx->dtask.p, c->ntask);
       HIPCHK(hipGetLastError());
     }
     Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
@@ -905,6 +930,10 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
                        ? launch_numc_dense<512, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr)
                        : launch_numc_dense<1024, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr);
     CHK(rc);
+    if (light_on(ctx, W, sizeof(V))) {
+      Phase phl(ctx, AMS_PH_NUMC_LIGHT, on);
+      CHK(launch_numc_light<V>(ctx, A, B, W, rp_c, col_c, val_c, cap));
+    }
   }
   if (A.m > 0) {
     Phase ph(ctx, AMS_PH_NUMC_HASH, on);
@@ -1104,6 +1133,7 @@ void ams_default_params(ams_params_t* p) {
   // row of the join-aggregate product goes 35 times slower there (740
   // against 21 ms)
   p->ghash_ratio = 0.25;
+  p->light_max = kLightMaxCnt;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -1184,6 +1214,8 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (p->esc_sort < 0 || p->esc_sort > 4) return fail(ctx, AMS_ERR_ARG, "esc_sort must be in [0, 4]");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
+  if (p->light_max < 0 || p->light_max > kLightMaxCnt)
+    return fail(ctx, AMS_ERR_ARG, "light_max must be in [0, %d]", kLightMaxCnt);
   ctx->prm = *p;
   return AMS_OK;
 }

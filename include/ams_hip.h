@@ -72,6 +72,9 @@ typedef struct {
 	double  ghash_ratio;    /* rows with flops > sym_hash_max and flops < ratio * words(n)
 	                           count and accumulate in global-memory hash tables
 	                           (synchronised calls; 0 = off)                      */
+	int32_t light_max;      /* dense-bin (row, column chunk) morsels of at most this many
+	                           C entries (and at most 256 A nonzeros) are accumulated by
+	                           one wave each in an LDS hash table; 0 = off, at most 341 */
 } ams_params_t;
 
 /* Counters of the last symbolic / numeric call (rows per bin, tasks). */
@@ -100,7 +103,8 @@ typedef struct {
 /* Phases timed with HIP events on the context stream when timing is on. */
 enum { AMS_PH_FLOPS = 0, AMS_PH_CLASSIFY = 1, AMS_PH_SYMB_HASH = 2, AMS_PH_SYMB_BITMAP = 3,
        AMS_PH_SCAN = 4, AMS_PH_BBLK = 5, AMS_PH_NUMC_DENSE = 6, AMS_PH_NUMC_HASH = 7,
-       AMS_PH_NUMC_ESC = 8, AMS_PH_SYMB_GHASH = 9, AMS_PH_NUMC_GHASH = 10, AMS_PH_COUNT = 11 };
+       AMS_PH_NUMC_ESC = 8, AMS_PH_SYMB_GHASH = 9, AMS_PH_NUMC_GHASH = 10, AMS_PH_NUMC_LIGHT = 11,
+       AMS_PH_COUNT = 12 };
 
 typedef struct ams_ctx ams_ctx_t;
 

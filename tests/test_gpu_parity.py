@@ -25,6 +25,12 @@ ROUTES = {
                       bitmap_bits=1024, dense_density=0.0),
     "dense_one_pass": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=4096,
                            bitmap_bits=65536, dense_density=0.0),
+    # every dense morsel through the block kernel (no light morsels)
+    "all_dense_block": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=128,
+                            bitmap_bits=1024, dense_density=0.0, light_max=0),
+    # morsels of up to 8 C entries through the wave kernel, the rest through the block kernel
+    "dense_light_mix": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=1024,
+                            bitmap_bits=65536, dense_density=0.0, light_max=8),
     "bitmap_then_hash": dict(sym_hash_max=8, num_hash_max=5461, bitmap_ratio=0.0, dense_density=1.0,
                              bitmap_bits=256, dense_width=64),
     "big_tables": dict(sym_hash_max=5461, num_hash_max=5461, acc_thres=0.0),
@@ -75,7 +81,8 @@ def test_rmat_square(gpu_ctx, route, name, a, exact):
     assert_matches_oracle(c, a, a, exact)
 
 
-@pytest.mark.parametrize("route", ["default", "all_dense", "esc_only", "blocked_sym_dense", "esc_bucket", "esc_merge",
+@pytest.mark.parametrize("route", ["default", "all_dense", "all_dense_block", "dense_light_mix", "esc_only",
+                                   "blocked_sym_dense", "esc_bucket", "esc_merge",
                                    "ghash"])
 def test_rectangular_unsorted(gpu_ctx, route):
     a = random_csr(70, 300, 0.05, seed=1)
