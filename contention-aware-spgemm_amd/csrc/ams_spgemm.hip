@@ -924,12 +924,14 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
                          (const int64_t*)flag, (DenseTask*)ctx->dtask.p, c->ntask);
       HIPCHK(hipGetLastError());
     }
-    Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
-    CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
-    const int rc = dense_block(W, sizeof(V)) == 512
-                       ? launch_numc_dense<512, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr)
-                       : launch_numc_dense<1024, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr);
-    CHK(rc);
+    {
+      Phase ph(ctx, AMS_PH_NUMC_DENSE, on);
+      CHK(fill_i64(ctx, (int64_t*)&c->ticket, 1, 0));
+      const int rc = dense_block(W, sizeof(V)) == 512
+                         ? launch_numc_dense<512, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr)
+                         : launch_numc_dense<1024, V>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr);
+      CHK(rc);
+    }
     if (light_on(ctx, W, sizeof(V))) {
       Phase phl(ctx, AMS_PH_NUMC_LIGHT, on);
       CHK(launch_numc_light<V>(ctx, A, B, W, rp_c, col_c, val_c, cap));
