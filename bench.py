@@ -99,7 +99,7 @@ def whole_bytes(st, v, nnz_a, m):
 
 KERNEL_OF_PHASE = {"numc_dense": "k_numc_dense", "symb_bitmap": "k_symb_bitmap", "numc_hash": "k_numc_hash",
                    "symb_hash": "k_symb_hash", "numc_esc": "k_numc_esc", "symb_ghash": "k_symb_ghash",
-                   "numc_ghash": "k_numc_ghash"}
+                   "numc_ghash": "k_numc_ghash", "numc_light": "k_numc_light"}
 
 
 def timed_steps(ctx, da, db, steps, warmup):
@@ -136,8 +136,14 @@ def roofline(tm, st, v, m, workload):
     bytes_dom = alg_bytes(dom, st, v, m)
     if bytes_dom is None or n_dom == 0:
         return None
-    ach = bytes_dom / (ms_dom / n_dom * 1e-3)
     kernel = KERNEL_OF_PHASE[dom]
+    light_ms, light_n = tm.get("numc_light", (0.0, 0))
+    if dom == "numc_dense" and light_ms > 0:
+        # the dense route's morsels are split between the block kernel and
+        # the light-morsel wave kernel: the route's bytes over both launches
+        ms_dom += light_ms * n_dom / max(light_n, 1)
+        kernel = "k_numc_dense+k_numc_light"
+    ach = bytes_dom / (ms_dom / n_dom * 1e-3)
     return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
             "traffic": read_traffic(kernel, workload), "kernel": kernel, "alg_bytes_per_launch": bytes_dom,
             "avg_launch_ms": ms_dom / n_dom}

@@ -2383,9 +2383,9 @@ struct DenseTask {
 };
 static_assert(sizeof(DenseTask) == 40, "descriptor dwords");
 
-// Light morsels: at most this many A nonzeros (a wave reads them 64 at a
-// time, so a long row stays with the block kernel).
-constexpr int kLightDeg = 256;
+// Light morsels: at most this many A nonzeros (two a lane of one wave; a
+// longer row stays with the block kernel).
+constexpr int kLightDeg = 128;
 
 // flag[c*H + h] for morsel (h, c) routed dense and not empty: 1 for the
 // block kernel, 1 << 32 for the wave kernel of light morsels (at most
@@ -2573,16 +2573,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
   unsigned long long t_cur = uniform_u64(s_ring[0]), t_nxt = uniform_u64(s_ring[1]);
   uint32_t dv = task_load(tasks, t_cur, ntask), ndv = task_load(tasks, t_nxt, ntask);
   DenseTask d = task_get(dv);
-  // this thread's segment of the current morsel's first window
-  int64_t st = 0;
-  int32_t len = 0;
+  // this thread's segment [st, en) of the current morsel's first window; its
+  // length is taken where it is first used, so that the load's wait comes
+  // there (after the previous morsel's output) and not right after the load
+  int64_t st = 0, en = 0;
   V a = V(0);
   if (t_cur < ntask && (int)threadIdx.x < d.deg) {
     const int32_t k = A.col[d.p0 + threadIdx.x];
     a = av[d.p0 + threadIdx.x];
     const int64_t* bp = bptr + (int64_t)d.c * ld;
     st = bp[k];
-    len = (int32_t)(bp[k + 1] - st);
+    en = bp[k + 1];
   }
   int pb = 0;       // pass bitmap of the next pass (three in turn)
   unsigned wc = 0;  // window counter: parity picks the scan scratch
@@ -2597,6 +2598,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     const uint32_t nndv = task_load(tasks, t_nn, ntask);  // descriptor two morsels ahead
     const int64_t p1 = d.p0 + d.deg;
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
+      const int32_t len = (int32_t)(en - st);
       // scan of the segment lengths: offsets and compact index
       int32_t off, ci, total;
       block_seg_scan<T>(len, s_w[wc++ & 1], off, ci, total);
@@ -2763,12 +2765,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
       }
       // the next window's B segment bounds (its A columns have arrived)
       st = 0;
-      len = 0;
+      en = 0;
       a = na;
       if (npre) {
         const int64_t* nbp = bptr + (int64_t)ncw * ld;
         st = nbp[nk];
-        len = (int32_t)(nbp[nk + 1] - st);
+        en = nbp[nk + 1];
       }
     }
     __syncthreads();  // every product of the morsel is in dense/bm
@@ -2880,34 +2882,43 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
 // set-up, three barriers, ranking): most of a G500 s19 ef32 row's 62 chunks
 // hold a few hundred products.  Here a wave takes such a morsel alone, with
 // no block barrier:
-//   * windows of 64 A nonzeros: B segment bounds of the chunk, a wave scan
-//     of their lengths, (bias, a) of the non-empty ones in a per-wave table;
+//   * the morsel's A nonzeros (at most kLightDeg, two a lane) and their B
+//     segments in the chunk: a wave scan of the lengths, (bias, a) of the
+//     non-empty segments in a per-wave table;
 //   * products in rounds of 64*kLightU positions: the segments starting in
 //     the round set bits in per-wave mark words, so position x belongs to
 //     segment (segments begun before the round) + (marks <= x) - 1;
-//   * each product a*b goes into a per-wave LDS hash table of kLightS slots
-//     (linear probing, claim by compare-and-swap; FP64 ds_add_f64, FP32 the
-//     compare-and-swap add) and its column into a chunk bitmap;
+//   * each product a*b goes into a per-wave LDS hash table (linear probing,
+//     claim by compare-and-swap, a lane's kLightU first probes issued
+//     together; FP64 ds_add_f64, FP32 the compare-and-swap add) of the
+//     fewest slots, 64 to kLightS, that the morsel's C entries fill at most
+//     two-thirds, and its column into a chunk bitmap;
 //   * output: prefix popcounts of the bitmap words give each present
 //     column its rank, so every slot stores its entry at rank order (sorted
 //     C without a sort); slots, bitmap and marks are cleared as they are
 //     read.
-// The next morsel's descriptor and first A entries are loaded while the
-// current one runs.  Per-wave dynamic LDS (light_wave_lds):
+// Morsels go through a three-stage pipeline: while morsel i runs, the B
+// segment bounds of morsel i+1, the A entries of morsel i+2 and the
+// descriptor of morsel i+3 (of the wave's stride) are in flight.  Per-wave
+// dynamic LDS (light_wave_lds):
 //   uint32 keys[kLightS] | V vals[kLightS] | uint32 bm[BW rounded to 4] |
-//   1 KB: SegBias<V> seg[64] (products), uint16 prefix[BW] (output) | u64 marks[kLightU]
+//   2 KB: SegBias<V> seg[kLightDeg] (products), uint16 prefix[BW] (output) | u64 marks[kLightU]
+#ifndef AMS_LIGHT_LG
+#define AMS_LIGHT_LG 9
+#endif
 constexpr int kLightT = 256;
-constexpr int kLightS = 512;
-constexpr int kLightLg = 9;
+constexpr int kLightLg = AMS_LIGHT_LG;
+constexpr int kLightS = 1 << kLightLg;
 constexpr int kLightMaxCnt = (kLightS * 2) / 3;
 constexpr int kLightMaxWords = 512;  // chunks of up to 16384 columns
 constexpr int kLightU = 4;
-static_assert((1 << kLightLg) == kLightS, "table size");
-static_assert(kLightMaxWords * 2 <= 64 * 16, "word prefixes fit the segment table's bytes");
+constexpr int kLightSegBytes = kLightDeg * 16;
+static_assert(kLightDeg == 2 * kWave, "two A nonzeros a lane");
+static_assert(kLightMaxWords * 2 <= kLightSegBytes, "word prefixes fit the segment table's bytes");
 
 __host__ __device__ constexpr size_t light_wave_lds(int W, int vbytes) {
-  return (size_t)kLightS * 4 + (size_t)kLightS * vbytes + (size_t)((((W + 31) >> 5) + 3) & ~3) * 4 + 1024 +
-         (size_t)kLightU * 8;
+  return (size_t)kLightS * 4 + (size_t)kLightS * vbytes + (size_t)((((W + 31) >> 5) + 3) & ~3) * 4 +
+         kLightSegBytes + (size_t)kLightU * 8;
 }
 __host__ __device__ constexpr size_t light_block_lds(int W, int vbytes) {
   return light_wave_lds(W, vbytes) * (kLightT / kWave);
@@ -2929,7 +2940,7 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
   unsigned char* tail = (unsigned char*)(bm + ((BW + 3) & ~3));
   SegBias<V>* seg = (SegBias<V>*)tail;
   uint16_t* pre = (uint16_t*)tail;
-  unsigned long long* marks = (unsigned long long*)(tail + 1024);
+  unsigned long long* marks = (unsigned long long*)(tail + kLightSegBytes);
   const int lane = lane_id();
   for (int i = lane; i < kLightS; i += kWave) {
     keys[i] = 0xffffffffu;
@@ -2945,60 +2956,97 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
   const int64_t nw = (int64_t)gridDim.x * (kLightT / kWave);
   int64_t t = (int64_t)blockIdx.x * (kLightT / kWave) + wave_id();
   if (t >= nlight) return;
-  DenseTask d = task_get(task_load(tasks, (unsigned long long)(nheavy + t), nend));
-  // this lane's A entry of the current window
-  bool has = lane < d.deg;
-  int32_t k = 0;
-  V a = V(0);
-  if (has) {
-    k = A.col[d.p0 + lane];
-    a = av[d.p0 + lane];
-  }
+  // descriptor of the wave's morsel t + i*nw (all zero past the end: deg 0)
+  auto desc = [&](int64_t i) {
+    const int64_t x = t + i * nw;
+    return task_load(tasks, (unsigned long long)(nheavy + (x < nlight ? x : nlight)), nend);
+  };
+  // a morsel's A entries 2*lane + s and their B segments in its chunk
+  auto load_a = [&](const DenseTask& m, int32_t (&k)[2], V (&a)[2]) {
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      const int e = 2 * lane + s2;
+      k[s2] = 0;
+      a[s2] = V(0);
+      if (e < m.deg) {
+        k[s2] = A.col[m.p0 + e];
+        a[s2] = av[m.p0 + e];
+      }
+    }
+  };
+  // (the segment [st, en); lengths are taken at use, so that no wait
+  // follows the load)
+  auto load_b = [&](const DenseTask& m, const int32_t (&k)[2], int64_t (&st)[2], int64_t (&en)[2]) {
+    const int64_t* bp = bptr + (int64_t)m.c * ld;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      st[s2] = 0;
+      en[s2] = 0;
+      if (2 * lane + s2 < m.deg) {
+        st[s2] = bp[k[s2]];
+        en[s2] = bp[k[s2] + 1];
+      }
+    }
+  };
+  // the morsel's table: 2^lg slots, the fewest that its C entries fill at
+  // most two-thirds
+  auto table_lg = [](int32_t cnt) {
+    int lg = 6;
+    while (lg < kLightLg && (1 << lg) * 2 < cnt * 3) ++lg;
+    return lg;
+  };
+  // stage set-up: morsel 0 with its B bounds, morsel 1 with its A entries,
+  // morsel 2's descriptor
+  DenseTask d0 = task_get(desc(0));
+  int32_t k0[2], k1[2];
+  V a0[2], a1[2];
+  int64_t st0[2], en0[2];
+  load_a(d0, k0, a0);
+  load_b(d0, k0, st0, en0);
+  DenseTask d1 = task_get(desc(1));
+  load_a(d1, k1, a1);
+  uint32_t dv2 = desc(2);
   for (; t < nlight; t += nw) {
-    const int64_t tn = t + nw;
-    const uint32_t ndv = task_load(tasks, (unsigned long long)(nheavy + tn), nend);
-    const int64_t* bp = bptr + (int64_t)d.c * ld;
-    const int64_t p1 = d.p0 + d.deg;
-    DenseTask nd = d;
-    for (int64_t w0 = d.p0;; w0 += kWave) {
-      int64_t st = 0;
-      int32_t len = 0;
-      if (has) {
-        st = bp[k];
-        len = (int32_t)(bp[k + 1] - st);
-      }
-      const V aw = a;
-      // the next window's A entries: this morsel's, or the next one's first
-      const bool more = w0 + kWave < p1;
-      int64_t np = 0;
-      int32_t nn = 0;
-      if (more) {
-        np = w0 + kWave;
-        nn = (int32_t)((p1 - np) < kWave ? (p1 - np) : kWave);
-      } else if (tn < nlight) {
-        nd = task_get(ndv);
-        np = nd.p0;
-        nn = nd.deg < kWave ? nd.deg : kWave;
-      }
-      has = lane < nn;
-      if (has) {
-        k = A.col[np + lane];
-        a = av[np + lane];
-      }
-      const int32_t incl = wave_incl_scan(len);
+    // in flight while this morsel runs: morsel 1's B bounds, morsel 2's A
+    // entries, morsel 3's descriptor
+    // (morsel 2's descriptor is read first: its wait then covers only
+    // loads issued a morsel ago)
+    const DenseTask d2 = task_get(dv2);
+    int64_t st1[2], en1[2];
+    load_b(d1, k1, st1, en1);
+    int32_t k2[2];
+    V a2[2];
+    load_a(d2, k2, a2);
+    const uint32_t dv3 = desc(3);
+    const DenseTask& d = d0;
+    const int lg = table_lg(d.cnt);
+    const uint32_t tmask = (1u << lg) - 1u;
+    {
+      int32_t ln0[2];
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) ln0[s2] = (int32_t)(en0[s2] - st0[s2]);
+      const int32_t lt = ln0[0] + ln0[1];
+      const int32_t incl = wave_incl_scan(lt);
       const int32_t total = wave_last(incl);
-      const int32_t off = incl - len;
-      const unsigned long long ne = __ballot(len > 0);
-      const uint32_t ci =
-          __builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
-      if (len > 0) seg[ci] = SegBias<V>{st - off, aw};
-      for (int32_t t0 = 0; t0 < total; t0 += kWave * kLightU) {
-        if (len > 0 && off >= t0 && off < t0 + kWave * kLightU) {
-          const int32_t x = off - t0;
-          atomicOr(&marks[x >> 6], 1ull << (x & 63));
+      int32_t off[2];
+      off[0] = incl - lt;
+      off[1] = off[0] + ln0[0];
+      const unsigned long long ne0 = __ballot(ln0[0] > 0), ne1 = __ballot(ln0[1] > 0);
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ne0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne0, 0u)) +
+                             __builtin_amdgcn_mbcnt_hi((uint32_t)(ne1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne1, 0u));
+      if (ln0[0] > 0) seg[below] = SegBias<V>{st0[0] - off[0], a0[0]};
+      if (ln0[1] > 0) seg[below + (ln0[0] > 0 ? 1 : 0)] = SegBias<V>{st0[1] - off[1], a0[1]};
+      for (int32_t r0 = 0; r0 < total; r0 += kWave * kLightU) {
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          if (ln0[s2] > 0 && off[s2] >= r0 && off[s2] < r0 + kWave * kLightU) {
+            const int32_t x = off[s2] - r0;
+            atomicOr(&marks[x >> 6], 1ull << (x & 63));
+          }
         }
         // non-empty segments begun before the round
-        const int32_t cov = (int32_t)__popcll(__ballot(len > 0 && off < t0));
+        const int32_t cov = (int32_t)(__popcll(__ballot(ln0[0] > 0 && off[0] < r0)) +
+                                      __popcll(__ballot(ln0[1] > 0 && off[1] < r0)));
         wave_lds_sync();
         unsigned long long mk[kLightU];
 #pragma unroll
@@ -3014,7 +3062,7 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
           int32_t cum = cov;
 #pragma unroll
           for (int u = 0; u < kLightU; ++u) {
-            const int32_t x = t0 + u * kWave + lane;
+            const int32_t x = r0 + u * kWave + lane;
             ok[u] = x < total;
             int32_t c = cum + (int32_t)__popcll(mk[u] & le) - 1;
             c = ok[u] ? c : 0;
@@ -3023,7 +3071,7 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
           }
 #pragma unroll
           for (int u = 0; u < kLightU; ++u) {
-            const int32_t x = t0 + u * kWave + lane;
+            const int32_t x = r0 + u * kWave + lane;
             const int64_t q = ok[u] ? sg[u].bias + (int64_t)x : 0;
             key[u] = bcol[q];
             pv[u] = bval[q];
@@ -3031,45 +3079,61 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
 #pragma unroll
           for (int u = 0; u < kLightU; ++u) pv[u] = sg[u].a * pv[u];
         }
+        // first probes of all slots at once, then the collisions one by one
+        uint32_t h[kLightU], old[kLightU];
+#pragma unroll
+        for (int u = 0; u < kLightU; ++u) {
+          h[u] = (key[u] * kHashMul) >> (32 - lg);
+          old[u] = ok[u] ? atomicCAS(&keys[h[u]], 0xffffffffu, key[u]) : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < kLightU; ++u) {
           if (!ok[u]) continue;
-          uint32_t h = (key[u] * kHashMul) >> (32 - kLightLg);
+          uint32_t hh = h[u], o = old[u];
           bool done = false;
-          for (int probe = 0; probe < kLightS; ++probe) {
-            const uint32_t old = atomicCAS(&keys[h], 0xffffffffu, key[u]);
-            if (old == 0xffffffffu) atomicOr(&bm[key[u] >> 5], 1u << (key[u] & 31));
-            if (old == 0xffffffffu || old == key[u]) {
-              lds_add(&vals[h], pv[u]);
+          for (uint32_t probe = 0; probe <= tmask; ++probe) {
+            if (o == 0xffffffffu) atomicOr(&bm[key[u] >> 5], 1u << (key[u] & 31));
+            if (o == 0xffffffffu || o == key[u]) {
+              lds_add(&vals[hh], pv[u]);
               done = true;
               break;
             }
-            h = (h + 1) & (kLightS - 1);
+            hh = (hh + 1) & tmask;
+            o = atomicCAS(&keys[hh], 0xffffffffu, key[u]);
           }
           if (!done) atomicOr(err, 0x10);
         }
       }
-      wave_lds_sync();  // the segment table is read before the next window writes it
-      if (!more) break;
+      wave_lds_sync();  // the segment table is read before the ranks overwrite it
     }
-    // ranks: prefix popcounts of the chunk bitmap's words (over the segment
-    // table, which the products no longer read)
-    int32_t run = 0;
-    for (int w0 = 0; w0 < BW; w0 += kWave) {
-      const int w = w0 + lane;
-      const uint32_t x = w < BW ? bm[w] : 0u;
-      const int32_t pc = __popc(x);
-      const int32_t inc = wave_incl_scan(pc);
-      if (w < BW) pre[w] = (uint16_t)(run + inc - pc);
-      run += wave_last(inc);
+    // ranks: lane l sums the popcounts of bitmap words [l*wpl, l*wpl + wpl),
+    // one wave scan, then each word's prefix goes over the segment table
+    const int wpl = (BW + kWave - 1) / kWave;
+    const int wb = lane * wpl;
+    int32_t sum = 0;
+    for (int q = 0; q < wpl; ++q)
+      if (wb + q < BW) sum += __popc(bm[wb + q]);
+    const int32_t inc = wave_incl_scan(sum);
+    int32_t run = inc - sum;
+    for (int q = 0; q < wpl; ++q) {
+      if (wb + q < BW) {
+        pre[wb + q] = (uint16_t)run;
+        run += __popc(bm[wb + q]);
+      }
     }
-    if (lane == 0 && run != d.cnt) atomicOr(err, 4);
+    if (lane == 0 && wave_last(inc) != d.cnt) atomicOr(err, 4);
     wave_lds_sync();
     int32_t* colp = col_c + d.off;
     V* valp = val_c + d.off;
-    for (int i = lane; i < kLightS; i += kWave) {
-      const uint32_t key = keys[i];
+    const int nu = 1 << (lg - 6);  // 64-slot rows of the morsel's table
+    uint32_t kx[kLightS / kWave];
+#pragma unroll
+    for (int u = 0; u < kLightS / kWave; ++u) kx[u] = u < nu ? keys[u * kWave + lane] : 0xffffffffu;
+#pragma unroll
+    for (int u = 0; u < kLightS / kWave; ++u) {
+      const uint32_t key = kx[u];
       if (key != 0xffffffffu) {
+        const int i = u * kWave + lane;
         const uint32_t wd = bm[key >> 5];
         const int32_t r = (int32_t)pre[key >> 5] + __popc(wd & ((1u << (key & 31)) - 1u));
         const V v = vals[i];
@@ -3084,7 +3148,18 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
     wave_lds_sync();  // every rank is read before the bitmap is cleared
     for (int i = lane; i < BW; i += kWave) bm[i] = 0u;
     wave_lds_sync();
-    d = nd;
+    // the stages move on (their loads have had this morsel's time)
+    d0 = d1;
+    d1 = d2;
+    dv2 = dv3;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      st0[s2] = st1[s2];
+      en0[s2] = en1[s2];
+      a0[s2] = a1[s2];
+      k1[s2] = k2[s2];
+      a1[s2] = a2[s2];
+    }
   }
 }
 

@@ -73,7 +73,7 @@ typedef struct {
 	                           count and accumulate in global-memory hash tables
 	                           (synchronised calls; 0 = off)                      */
 	int32_t light_max;      /* dense-bin (row, column chunk) morsels of at most this many
-	                           C entries (and at most 256 A nonzeros) are accumulated by
+	                           C entries (and at most 128 A nonzeros) are accumulated by
 	                           one wave each in an LDS hash table; 0 = off, at most 341 */
 } ams_params_t;
 
