@@ -58,6 +58,8 @@ struct Probs {
   int64_t max_width;      // max output columns of one problem
   const int2* arow_cols;  // per A row: (column base, width) of its problem, or null
   const int32_t* brow_lo; // per B row: column base of its problem, or null
+  uint32_t* rowbits;      // column bitmaps of the bitmap bin's rows (row h at h * bwords), or null
+  int64_t bwords;         // words per row in rowbits
 };
 
 __device__ __forceinline__ int find_prob(const int64_t* off, int n, int64_t r) {

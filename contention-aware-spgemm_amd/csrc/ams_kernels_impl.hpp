@@ -717,6 +717,10 @@ __global__ __launch_bounds__(T) void k_symb_bitmap_wave(Csr A, Csr B, Probs pr, 
                      if (!(bm[u >> 5] & bit)) atomicOr(&bm[u >> 5], bit);
                    });
     wave_lds_sync();
+    if (pr.rowbits) {
+      uint32_t* rb = pr.rowbits + (int64_t)h * pr.bwords;
+      for (int w = lane_id(); w < words; w += kWave) rb[w] = bm[w];
+    }
     int32_t* crow = coff + (int64_t)h * (R + 1);
     const int nch = (int)((width + W - 1) / W);
     int32_t running = 0;
@@ -817,8 +821,11 @@ __global__ __launch_bounds__(T) void k_heavy_finish(const Probs pr, const int32_
       const int64_t w0 = (int64_t)c * wchunk;
       const int64_t w1 = (w0 + wchunk) < words ? (w0 + wchunk) : words;
       int64_t pc = 0;
+      uint32_t* rb = pr.rowbits ? pr.rowbits + (int64_t)h * pr.bwords : nullptr;
       for (int64_t w = w0 + threadIdx.x; w < w1; w += T) {
-        pc += __popc(g[w]);
+        const uint32_t x = g[w];
+        pc += __popc(x);
+        if (rb) rb[w] = x;
         g[w] = 0u;
       }
       const int64_t tot = block_sum<T, int64_t>(pc, s_w);
@@ -973,6 +980,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       __syncthreads();
       const int64_t base = lo + c0;
       symb_bitmap_expand<T, Q32>(A.rp[row], A.rp[row + 1], L, seg, B, bm, base, pw);
+      if (pr.rowbits) {  // the row's column bitmap, for the numeric phase
+        uint32_t* rb = pr.rowbits + (int64_t)h * pr.bwords + (c0 >> 5);
+        for (int w = threadIdx.x; w < words; w += T) rb[w] = bm[w];
+      }
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {
         const int w = w0 + threadIdx.x;
@@ -2372,7 +2383,7 @@ __global__ __launch_bounds__(256) void k_bblk_scatter(Csr B, Probs pr, int W, in
 
 // One morsel of the dense bin: wide row `row`, column chunk `c`.
 struct DenseTask {
-  int32_t row;
+  int32_t h;        // the row's index in the bitmap bin (its coff and rowbits row)
   int32_t cnt;      // nnz of C in this chunk
   int32_t c;
   int32_t deg;      // nnz of A's row
@@ -2431,7 +2442,7 @@ __global__ __launch_bounds__(256) void k_dense_fill(Csr A, Probs pr, int W, cons
     const int32_t row = list[start + h];
     const int32_t* crow = coff + (int64_t)h * (R + 1);
     DenseTask d;
-    d.row = row;
+    d.h = h;
     d.cnt = crow[c + 1] - crow[c];
     d.c = c;
     d.p0 = A.rp[row];
@@ -2474,7 +2485,7 @@ __device__ __forceinline__ uint32_t task_load(const DenseTask* tasks, unsigned l
 __device__ __forceinline__ DenseTask task_get(uint32_t v) {
   auto lane = [&](int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); };
   DenseTask d;
-  d.row = (int32_t)lane(0);
+  d.h = (int32_t)lane(0);
   d.cnt = (int32_t)lane(1);
   d.c = (int32_t)lane(2);
   d.deg = (int32_t)lane(3);
@@ -2516,7 +2527,11 @@ struct DenseItem {
 #ifndef AMS_DENSE_WPE
 #define AMS_DENSE_WPE 4
 #endif
-template <int T, typename V>
+// PB: the symbolic phase left each row's column bitmap in pr.rowbits (one
+// word a thread: W <= 32 T): the products skip the bitmap test and OR, and
+// each morsel's words are loaded while its products run and stored into the
+// LDS bitmap before its output.
+template <int T, typename V, bool PB>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? AMS_DENSE_WPE : 4))) void k_numc_dense(
                                                   Csr A, int64_t mB, const int64_t* bptr,
                                                   const uint16_t* bcol, const V* bval,
@@ -2596,6 +2611,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     const unsigned long long t_nn = uniform_u64(s_ring[(mc + 2) & 3]);
     if (threadIdx.x == 0) s_ring[(mc + 3) & 3] = take();
     const uint32_t nndv = task_load(tasks, t_nn, ntask);  // descriptor two morsels ahead
+    // the morsel's column bitmap word of this thread (PB)
+    uint32_t pbw = 0u;
+    if constexpr (PB) {
+      if ((int)threadIdx.x < ((d.cwidth + 31) >> 5))
+        pbw = pr.rowbits[(int64_t)d.h * pr.bwords + (int64_t)d.c * (W >> 5) + threadIdx.x];
+    }
     const int64_t p1 = d.p0 + d.deg;
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
       const int32_t len = (int32_t)(en - st);
@@ -2732,7 +2753,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
               for (int j = 0; j < G; ++j) {
                 kk[j] = ok[g0 + j] ? item[g0 + j].key : (uint32_t)(W + lane_id());
                 atomicAdd(&dense[kk[j]], item[g0 + j].a * item[g0 + j].val);
-                bw[j] = bm[kk[j] >> 5];
+                if constexpr (!PB) bw[j] = bm[kk[j] >> 5];
               }
             } else {  // the group's FP32 swaps back to back
               V pv[G];
@@ -2742,13 +2763,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
                 pv[j] = item[g0 + j].a * item[g0 + j].val;
               }
               lds_add_group<G>(dense, kk, pv);
+              if constexpr (!PB) {
 #pragma unroll
-              for (int j = 0; j < G; ++j) bw[j] = bm[kk[j] >> 5];
+                for (int j = 0; j < G; ++j) bw[j] = bm[kk[j] >> 5];
+              }
             }
+            if constexpr (!PB) {
 #pragma unroll
-            for (int j = 0; j < G; ++j) {
-              const uint32_t bit = 1u << (kk[j] & 31);
-              if (!(bw[j] & bit)) atomicOr(&bm[kk[j] >> 5], bit);
+              for (int j = 0; j < G; ++j) {
+                const uint32_t bit = 1u << (kk[j] & 31);
+                if (!(bw[j] & bit)) atomicOr(&bm[kk[j] >> 5], bit);
+              }
             }
           };
           issue(std::integral_constant<int, 0>{});
@@ -2772,6 +2797,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
         st = nbp[nk];
         en = nbp[nk + 1];
       }
+    }
+    if constexpr (PB) {
+      if ((int)threadIdx.x < ((d.cwidth + 31) >> 5)) bm[threadIdx.x] = pbw;
     }
     __syncthreads();  // every product of the morsel is in dense/bm
     // rank and emit by waves: wave w owns words [w*wpw, w*wpw + wpw) of this

@@ -143,6 +143,8 @@ struct ams_ctx {
   Buf hflag, hrow, hpiece, gbm;  // heavy bitmap rows: flags, slots, pieces, global bitmaps
   Buf gtab, gnum;                // global-hash tables: symbolic keys, numeric regions
   Buf bgrad;                     // backward: dB partials of the dense bin in blocked-B order
+  Buf rowbits;                   // column bitmaps of the bitmap bin's rows, kept for the numeric phase
+  size_t mem_total = 0;          // the GPU's memory
   int64_t ghash_num_bytes = 0;   // of the last symbolic plan
   size_t gbm_zero = 0;           // bytes of gbm known to be zero
   bool bblk_ready = false;  // blocked B of the current plan already built
@@ -598,8 +600,11 @@ static int reroute_numeric(ams_ctx_t* ctx, const ams_dcsr_t* a, const int64_t* r
 }
 
 static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype,
-                        const Probs& pr, int64_t* rp_c, bool sync, int64_t* nnz) {
+                        const Probs& pr_in, int64_t* rp_c, bool sync, int64_t* nnz) {
   const Csr A = to_csr(a), B = to_csr(b);
+  Probs pr = pr_in;
+  pr.rowbits = nullptr;
+  pr.bwords = 0;
   const int64_t m = A.m;
   ctx->plan_valid = false;
   CHK(grow(ctx, ctx->ctrl, sizeof(Ctrl)));
@@ -656,6 +661,20 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   const int64_t H = ctx->plan_counts ? ctx->cnt_sym[SB_BITMAP] : m;
   ctx->plan_H = H;
   CHK(grow(ctx, ctx->coff, sizeof(int32_t) * std::max<int64_t>(H, 1) * (R + 1)));
+  // The bitmap bin's column bitmaps are kept for the numeric phase (its
+  // dense chunks then skip the per-product bitmap test and OR) in
+  // synchronised calls of the row-bitmap symbolic, when they take at most a
+  // twelfth of the GPU's memory (G500 s19 ef32: 20 GB next to C's 246 GB).
+  if (ctx->plan_counts && !ctx->fixed && !ctx->prm.sym_blocked && H > 0 && ctx->mem_total > 0) {
+    const int64_t bwords = (std::max<int64_t>(pr.max_width, 1) + 31) / 32;
+    const size_t bytes = (size_t)H * (size_t)bwords * 4;
+    if (bytes <= ctx->mem_total / 12) {
+      CHK(grow(ctx, ctx->rowbits, bytes));
+      pr.rowbits = (uint32_t*)ctx->rowbits.p;
+      pr.bwords = bwords;
+    }
+  }
+  ctx->plan_pr = pr;
 
   if (m > 0) {
     int64_t* rownz = rp_c;
@@ -846,23 +865,33 @@ static void launch_numc_esc(ams_ctx_t* ctx, const Csr& A, const Csr& B, const in
   else launch_numc_esc_k<EP, V, false>(ctx, A, B, rp_c, col_c, val_c, cap, bin, cnt);
 }
 
-template <int T, typename V>
-static int launch_numc_dense(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, int64_t nall, const int64_t* rp_c,
-                             int32_t* col_c, V* val_c, int64_t cap, const Probs& pr) {
+template <int T, typename V, bool PB>
+static int launch_numc_dense_pb(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, int64_t nall,
+                                const int64_t* rp_c, int32_t* col_c, V* val_c, int64_t cap, const Probs& pr) {
   Ctrl* c = dctrl(ctx);
   const size_t dyn = dense_lds<T>(W, sizeof(V));
   if (dyn + dense_static_lds<T>() > ctx->lds_max)
     return fail(ctx, AMS_ERR_ARG, "dense_width %d needs %zu B of LDS", W, dyn + dense_static_lds<T>());
-  HIPCHK(hipFuncSetAttribute((const void*)k_numc_dense<T, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIPCHK(hipFuncSetAttribute((const void*)k_numc_dense<T, V, PB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)dyn));
   int occ;
-  occupancy(k_numc_dense<T, V>, T, dyn, &occ);
+  occupancy(k_numc_dense<T, V, PB>, T, dyn, &occ);
   const int g = grid_for(ctx, occ, nall, ctx->plan_counts);
-  hipLaunchKernelGGL((k_numc_dense<T, V>), dim3(g), dim3(T), dyn, ctx->stream, A, B.m, (const int64_t*)ctx->bptr.p,
-                     (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p, (const DenseTask*)ctx->dtask.p,
-                     (const int64_t*)&c->ntask[0], W, rp_c, col_c, val_c, cap, pr, &c->ticket, &c->err);
+  hipLaunchKernelGGL((k_numc_dense<T, V, PB>), dim3(g), dim3(T), dyn, ctx->stream, A, B.m,
+                     (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
+                     (const DenseTask*)ctx->dtask.p, (const int64_t*)&c->ntask[0], W, rp_c, col_c, val_c, cap, pr,
+                     &c->ticket, &c->err);
   HIPCHK(hipGetLastError());
   return AMS_OK;
+}
+
+// The kept row bitmaps serve when one word a thread covers a chunk.
+template <int T, typename V>
+static int launch_numc_dense(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, int64_t nall, const int64_t* rp_c,
+                             int32_t* col_c, V* val_c, int64_t cap, const Probs& pr) {
+  if (pr.rowbits && (W + 31) / 32 <= T)
+    return launch_numc_dense_pb<T, V, true>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr);
+  return launch_numc_dense_pb<T, V, false>(ctx, A, B, W, nall, rp_c, col_c, val_c, cap, pr);
 }
 
 // Light morsels go to the wave kernel when light_max > 0 and a wave's chunk
@@ -1152,6 +1181,7 @@ int ams_ctx_create(ams_ctx_t** out, void* stream) {
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) {
     ctx->num_cu = prop.multiProcessorCount;
     ctx->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+    ctx->mem_total = prop.totalGlobalMem;
   }
   if (hipHostMalloc((void**)&ctx->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_pin, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
@@ -1180,7 +1210,7 @@ int ams_ctx_destroy(ams_ctx_t* ctx) {
                  &ctx->bptr, &ctx->bcol, &ctx->bval, &ctx->bperm, &ctx->tsum, &ctx->pool_rp, &ctx->pool_col,
                  &ctx->pool_val, &ctx->probs, &ctx->scratch_a, &ctx->dflag, &ctx->dtask, &ctx->ccnt,
                  &ctx->prow, &ctx->pbrow, &ctx->lrows, &ctx->hflag, &ctx->hrow, &ctx->hpiece, &ctx->gbm,
-                 &ctx->gtab, &ctx->gnum, &ctx->bgrad};
+                 &ctx->gtab, &ctx->gnum, &ctx->bgrad, &ctx->rowbits};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : ctx->recs) {
