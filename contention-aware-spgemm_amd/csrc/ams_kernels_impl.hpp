@@ -2944,39 +2944,62 @@ constexpr int kLightSegBytes = kLightDeg * 16;
 static_assert(kLightDeg == 2 * kWave, "two A nonzeros a lane");
 static_assert(kLightMaxWords * 2 <= kLightSegBytes, "word prefixes fit the segment table's bytes");
 
-__host__ __device__ constexpr size_t light_wave_lds(int W, int vbytes) {
-  return (size_t)kLightS * 4 + (size_t)kLightS * vbytes + (size_t)((((W + 31) >> 5) + 3) & ~3) * 4 +
-         kLightSegBytes + (size_t)kLightU * 8;
+// PB (the symbolic phase kept the rows' column bitmaps): no hash table; the
+// wave loads its chunk's bitmap words and their rank prefixes into a uint2
+// array, and each product adds at its column's rank:
+//   uint2 wp[BW rounded to 2] | V vals[kLightS] | 2 KB segment table | u64 marks[kLightU]
+__host__ __device__ constexpr size_t light_wave_lds(int W, int vbytes, bool pb) {
+  return pb ? (size_t)((((W + 31) >> 5) + 1) & ~1) * 8 + (size_t)kLightS * vbytes + kLightSegBytes +
+                  (size_t)kLightU * 8
+            : (size_t)kLightS * 4 + (size_t)kLightS * vbytes + (size_t)((((W + 31) >> 5) + 3) & ~3) * 4 +
+                  kLightSegBytes + (size_t)kLightU * 8;
 }
-__host__ __device__ constexpr size_t light_block_lds(int W, int vbytes) {
-  return light_wave_lds(W, vbytes) * (kLightT / kWave);
+__host__ __device__ constexpr size_t light_block_lds(int W, int vbytes, bool pb) {
+  return light_wave_lds(W, vbytes, pb) * (kLightT / kWave);
 }
+constexpr int kLightPbWpl = kLightMaxWords / kWave;  // bitmap words a lane (PB)
 
-template <typename V>
+template <typename V, bool PB>
 __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const int64_t* bptr,
                                                         const uint16_t* bcol, const V* bval,
                                                         const DenseTask* tasks, const int64_t* __restrict__ d_ntask,
                                                         int W, const int64_t* rp_c, int32_t* col_c, V* val_c,
-                                                        int64_t cap, int32_t* err) {
+                                                        int64_t cap, Probs pr, int32_t* err) {
   extern __shared__ __align__(16) unsigned char dyn_raw[];
   if (rp_c[A.m] > cap) return;
   const int BW = (W + 31) >> 5;
-  unsigned char* mine = dyn_raw + light_wave_lds(W, (int)sizeof(V)) * wave_id();
+  unsigned char* mine = dyn_raw + light_wave_lds(W, (int)sizeof(V), PB) * wave_id();
+  // !PB: keys | vals | bm | tail; PB: wp | vals | tail
   uint32_t* keys = (uint32_t*)mine;
-  V* vals = (V*)(keys + kLightS);
+  uint2* wp = (uint2*)mine;
+  V* vals = PB ? (V*)(wp + ((BW + 1) & ~1)) : (V*)(keys + kLightS);
   uint32_t* bm = (uint32_t*)(vals + kLightS);
-  unsigned char* tail = (unsigned char*)(bm + ((BW + 3) & ~3));
+  unsigned char* tail = PB ? (unsigned char*)(vals + kLightS) : (unsigned char*)(bm + ((BW + 3) & ~3));
   SegBias<V>* seg = (SegBias<V>*)tail;
   uint16_t* pre = (uint16_t*)tail;
   unsigned long long* marks = (unsigned long long*)(tail + kLightSegBytes);
   const int lane = lane_id();
   for (int i = lane; i < kLightS; i += kWave) {
-    keys[i] = 0xffffffffu;
+    if constexpr (!PB) keys[i] = 0xffffffffu;
     vals[i] = V(0);
   }
-  for (int i = lane; i < BW; i += kWave) bm[i] = 0u;
+  if constexpr (!PB) {
+    for (int i = lane; i < BW; i += kWave) bm[i] = 0u;
+  }
   if (lane < kLightU) marks[lane] = 0ull;
   wave_lds_sync();
+  // PB: lane l holds the chunk's bitmap words [l*wpl, l*wpl + wpl)
+  const int wpl = (BW + kWave - 1) / kWave;
+  auto load_w = [&](const DenseTask& m, uint32_t (&ww)[kLightPbWpl]) {
+    const int mw = (m.cwidth + 31) >> 5;
+    const uint32_t* rb = pr.rowbits + (int64_t)m.h * pr.bwords + (int64_t)m.c * (W >> 5);
+#pragma unroll
+    for (int q = 0; q < kLightPbWpl; ++q) {
+      const int w = lane * wpl + q;
+      ww[q] = 0u;
+      if (q < wpl && w < mw && m.deg > 0) ww[q] = rb[w];
+    }
+  };
   const int64_t nheavy = d_ntask[0], nlight = d_ntask[1];
   const unsigned long long nend = (unsigned long long)(nheavy + nlight);
   const int64_t ld = mB + 1;
@@ -3031,6 +3054,8 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
   int64_t st0[2], en0[2];
   load_a(d0, k0, a0);
   load_b(d0, k0, st0, en0);
+  uint32_t ww0[kLightPbWpl], ww1[kLightPbWpl];
+  if constexpr (PB) load_w(d0, ww0);
   DenseTask d1 = task_get(desc(1));
   load_a(d1, k1, a1);
   uint32_t dv2 = desc(2);
@@ -3042,6 +3067,7 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
     const DenseTask d2 = task_get(dv2);
     int64_t st1[2], en1[2];
     load_b(d1, k1, st1, en1);
+    if constexpr (PB) load_w(d1, ww1);
     int32_t k2[2];
     V a2[2];
     load_a(d2, k2, a2);
@@ -3049,6 +3075,23 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
     const DenseTask& d = d0;
     const int lg = table_lg(d.cnt);
     const uint32_t tmask = (1u << lg) - 1u;
+    int32_t nbits = 0;  // PB: set bits of the chunk
+    if constexpr (PB) {
+      // each word's rank prefix: a wave scan of the lanes' popcounts
+      int32_t sum = 0;
+#pragma unroll
+      for (int q = 0; q < kLightPbWpl; ++q) sum += __popc(ww0[q]);
+      const int32_t inc = wave_incl_scan(sum);
+      int32_t run = inc - sum;
+#pragma unroll
+      for (int q = 0; q < kLightPbWpl; ++q) {
+        const int w = lane * wpl + q;
+        if (q < wpl && w < BW) wp[w] = make_uint2(ww0[q], (uint32_t)run);
+        run += __popc(ww0[q]);
+      }
+      nbits = wave_last(inc);
+      wave_lds_sync();
+    }
     {
       int32_t ln0[2];
 #pragma unroll
@@ -3107,6 +3150,17 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
 #pragma unroll
           for (int u = 0; u < kLightU; ++u) pv[u] = sg[u].a * pv[u];
         }
+        if constexpr (PB) {  // each product adds at its column's rank
+          uint2 wr[kLightU];
+#pragma unroll
+          for (int u = 0; u < kLightU; ++u) wr[u] = wp[ok[u] ? key[u] >> 5 : 0];
+#pragma unroll
+          for (int u = 0; u < kLightU; ++u) {
+            const uint32_t r = wr[u].y + __popc(wr[u].x & ((1u << (key[u] & 31)) - 1u));
+            if (ok[u] && r < (uint32_t)kLightS) lds_add(&vals[r], pv[u]);
+          }
+          continue;
+        }
         // first probes of all slots at once, then the collisions one by one
         uint32_t h[kLightU], old[kLightU];
 #pragma unroll
@@ -3134,9 +3188,50 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
       }
       wave_lds_sync();  // the segment table is read before the ranks overwrite it
     }
+    if constexpr (PB) {
+      // values are at their ranks already; the columns from the bitmap words
+      if (lane == 0 && nbits != d.cnt) atomicOr(err, 4);
+      int32_t* colp = col_c + d.off;
+      V* valp = val_c + d.off;
+      for (int i = lane; i < d.cnt; i += kWave) {
+        valp[i] = vals[i];
+        vals[i] = V(0);
+      }
+      int32_t r = 0;
+      {
+        int32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kLightPbWpl; ++q) sum += __popc(ww0[q]);
+        r = wave_incl_scan(sum) - sum;
+      }
+#pragma unroll
+      for (int q = 0; q < kLightPbWpl; ++q) {
+        uint32_t x = ww0[q];
+        const int32_t cb = d.colbase + (lane * wpl + q) * 32;
+        while (x) {
+          if (r < d.cnt) colp[r] = cb + __builtin_ctz(x);
+          ++r;
+          x &= x - 1u;
+        }
+      }
+      wave_lds_sync();  // vals and wp are read before the next morsel writes them
+      d0 = d1;
+      d1 = d2;
+      dv2 = dv3;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        st0[s2] = st1[s2];
+        en0[s2] = en1[s2];
+        a0[s2] = a1[s2];
+        k1[s2] = k2[s2];
+        a1[s2] = a2[s2];
+      }
+#pragma unroll
+      for (int q = 0; q < kLightPbWpl; ++q) ww0[q] = ww1[q];
+      continue;
+    }
     // ranks: lane l sums the popcounts of bitmap words [l*wpl, l*wpl + wpl),
     // one wave scan, then each word's prefix goes over the segment table
-    const int wpl = (BW + kWave - 1) / kWave;
     const int wb = lane * wpl;
     int32_t sum = 0;
     for (int q = 0; q < wpl; ++q)

@@ -897,26 +897,45 @@ static int launch_numc_dense(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, 
 // Light morsels go to the wave kernel when light_max > 0 and a wave's chunk
 // bitmap fits its share of the LDS.
 static bool light_on(const ams_ctx_t* ctx, int W, int vbytes) {
+  const bool pb = ctx->plan_pr.rowbits != nullptr;
   return ctx->prm.light_max > 0 && (W + 31) / 32 <= kLightMaxWords &&
-         light_block_lds(W, vbytes) <= ctx->lds_max;
+         light_block_lds(W, vbytes, pb) <= ctx->lds_max;
 }
 
+// C entries a light morsel may have: up to kLightS at ranks from the kept
+// row bitmaps, up to kLightMaxCnt (a table two-thirds full) in hash tables.
+static int light_limit(const ams_ctx_t* ctx, int W, int vbytes) {
+  if (!light_on(ctx, W, vbytes)) return 0;
+  return std::min(ctx->prm.light_max, ctx->plan_pr.rowbits ? kLightS : kLightMaxCnt);
+}
+
+template <typename V, bool PB>
+static int launch_numc_light_pb(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, const int64_t* rp_c,
+                                int32_t* col_c, V* val_c, int64_t cap) {
+  Ctrl* c = dctrl(ctx);
+  const size_t dyn = light_block_lds(W, sizeof(V), PB);
+  HIPCHK(hipFuncSetAttribute((const void*)k_numc_light<V, PB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)dyn));
+  int occ;
+  occupancy(k_numc_light<V, PB>, kLightT, dyn, &occ);
+  // persistent waves: each walks the light morsels with the grid's stride
+  const int g = std::max(1, occ) * ctx->num_cu;
+  hipLaunchKernelGGL((k_numc_light<V, PB>), dim3(g), dim3(kLightT), dyn, ctx->stream, A, B.m,
+                     (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
+                     (const DenseTask*)ctx->dtask.p, (const int64_t*)c->ntask, W, rp_c, col_c, val_c, cap,
+                     ctx->plan_pr, &c->err);
+  HIPCHK(hipGetLastError());
+  return AMS_OK;
+}
+
+// With the symbolic phase's row bitmaps the light morsels add at their
+// columns' ranks; without them, in per-wave hash tables.
 template <typename V>
 static int launch_numc_light(ams_ctx_t* ctx, const Csr& A, const Csr& B, int W, const int64_t* rp_c,
                              int32_t* col_c, V* val_c, int64_t cap) {
-  Ctrl* c = dctrl(ctx);
-  const size_t dyn = light_block_lds(W, sizeof(V));
-  HIPCHK(hipFuncSetAttribute((const void*)k_numc_light<V>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-  int occ;
-  occupancy(k_numc_light<V>, kLightT, dyn, &occ);
-  // persistent waves: each walks the light morsels with the grid's stride
-  const int g = std::max(1, occ) * ctx->num_cu;
-  hipLaunchKernelGGL((k_numc_light<V>), dim3(g), dim3(kLightT), dyn, ctx->stream, A, B.m,
-                     (const int64_t*)ctx->bptr.p, (const uint16_t*)ctx->bcol.p, (const V*)ctx->bval.p,
-                     (const DenseTask*)ctx->dtask.p, (const int64_t*)c->ntask, W, rp_c, col_c, val_c, cap,
-                     &c->err);
-  HIPCHK(hipGetLastError());
-  return AMS_OK;
+  if (ctx->plan_pr.rowbits)
+    return launch_numc_light_pb<V, true>(ctx, A, B, W, rp_c, col_c, val_c, cap);
+  return launch_numc_light_pb<V, false>(ctx, A, B, W, rp_c, col_c, val_c, cap);
 }
 
 template <typename V>
@@ -945,8 +964,7 @@ static int run_numeric_t(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* 
       const int gf = (int)std::max<int64_t>(1, std::min<int64_t>((nall + 255) / 256, (int64_t)ctx->num_cu * 16));
       hipLaunchKernelGGL(k_dense_flags, dim3(gf), dim3(256), 0, ctx->stream, (const int32_t*)ctx->lsym.p,
                          (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const uint8_t*)ctx->nbin.p,
-                         (const int32_t*)ctx->coff.p, R, A.rp, light_on(ctx, W, sizeof(V)) ? ctx->prm.light_max : 0,
-                         flag);
+                         (const int32_t*)ctx->coff.p, R, A.rp, light_limit(ctx, W, sizeof(V)), flag);
       CHK(scan_i64(ctx, flag, flag, nall + 1));
       hipLaunchKernelGGL(k_dense_fill, dim3(gf), dim3(256), 0, ctx->stream, A, pr, W, (const int32_t*)ctx->lsym.p,
                          (const int32_t*)c->cnt_sym, (int)SB_BITMAP, (const int32_t*)ctx->coff.p, R, rp_c,
@@ -1164,7 +1182,10 @@ void ams_default_params(ams_params_t* p) {
   // row of the join-aggregate product goes 35 times slower there (740
   // against 21 ms)
   p->ghash_ratio = 0.25;
-  p->light_max = kLightMaxCnt;
+  // light morsels: at most 512 C entries at ranks from the kept row
+  // bitmaps (341 in hash tables); G500 s19 ef32 dense route 243.2 ms at 341,
+  // 237.7 ms at 512
+  p->light_max = kLightS;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -1246,8 +1267,8 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (p->esc_sort < 0 || p->esc_sort > 4) return fail(ctx, AMS_ERR_ARG, "esc_sort must be in [0, 4]");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
-  if (p->light_max < 0 || p->light_max > kLightMaxCnt)
-    return fail(ctx, AMS_ERR_ARG, "light_max must be in [0, %d]", kLightMaxCnt);
+  if (p->light_max < 0 || p->light_max > kLightS)
+    return fail(ctx, AMS_ERR_ARG, "light_max must be in [0, %d]", kLightS);
   ctx->prm = *p;
   return AMS_OK;
 }

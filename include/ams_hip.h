@@ -74,7 +74,9 @@ typedef struct {
 	                           (synchronised calls; 0 = off)                      */
 	int32_t light_max;      /* dense-bin (row, column chunk) morsels of at most this many
 	                           C entries (and at most 128 A nonzeros) are accumulated by
-	                           one wave each in an LDS hash table; 0 = off, at most 341 */
+	                           one wave each: at their columns' ranks when the symbolic
+	                           phase kept the row bitmaps (up to 512 entries), else in an
+	                           LDS hash table (up to 341); 0 = off, at most 512 */
 } ams_params_t;
 
 /* Counters of the last symbolic / numeric call (rows per bin, tasks). */
