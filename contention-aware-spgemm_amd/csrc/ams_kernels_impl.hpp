@@ -2934,9 +2934,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
 #ifndef AMS_LIGHT_LG
 #define AMS_LIGHT_LG 9
 #endif
+#ifndef AMS_LIGHT_CAP_LG
+#define AMS_LIGHT_CAP_LG 9
+#endif
 constexpr int kLightT = 256;
 constexpr int kLightLg = AMS_LIGHT_LG;
 constexpr int kLightS = 1 << kLightLg;
+constexpr int kLightCap = 1 << AMS_LIGHT_CAP_LG;  // values of a rank-addressed (PB) light morsel
 constexpr int kLightMaxCnt = (kLightS * 2) / 3;
 constexpr int kLightMaxWords = 512;  // chunks of up to 16384 columns
 constexpr int kLightU = 4;
@@ -2947,9 +2951,9 @@ static_assert(kLightMaxWords * 2 <= kLightSegBytes, "word prefixes fit the segme
 // PB (the symbolic phase kept the rows' column bitmaps): no hash table; the
 // wave loads its chunk's bitmap words and their rank prefixes into a uint2
 // array, and each product adds at its column's rank:
-//   uint2 wp[BW rounded to 2] | V vals[kLightS] | 2 KB segment table | u64 marks[kLightU]
+//   uint2 wp[BW rounded to 2] | V vals[kLightCap] | 2 KB segment table | u64 marks[kLightU]
 __host__ __device__ constexpr size_t light_wave_lds(int W, int vbytes, bool pb) {
-  return pb ? (size_t)((((W + 31) >> 5) + 1) & ~1) * 8 + (size_t)kLightS * vbytes + kLightSegBytes +
+  return pb ? (size_t)((((W + 31) >> 5) + 1) & ~1) * 8 + (size_t)kLightCap * vbytes + kLightSegBytes +
                   (size_t)kLightU * 8
             : (size_t)kLightS * 4 + (size_t)kLightS * vbytes + (size_t)((((W + 31) >> 5) + 3) & ~3) * 4 +
                   kLightSegBytes + (size_t)kLightU * 8;
@@ -2974,12 +2978,12 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
   uint2* wp = (uint2*)mine;
   V* vals = PB ? (V*)(wp + ((BW + 1) & ~1)) : (V*)(keys + kLightS);
   uint32_t* bm = (uint32_t*)(vals + kLightS);
-  unsigned char* tail = PB ? (unsigned char*)(vals + kLightS) : (unsigned char*)(bm + ((BW + 3) & ~3));
+  unsigned char* tail = PB ? (unsigned char*)(vals + kLightCap) : (unsigned char*)(bm + ((BW + 3) & ~3));
   SegBias<V>* seg = (SegBias<V>*)tail;
   uint16_t* pre = (uint16_t*)tail;
   unsigned long long* marks = (unsigned long long*)(tail + kLightSegBytes);
   const int lane = lane_id();
-  for (int i = lane; i < kLightS; i += kWave) {
+  for (int i = lane; i < (PB ? kLightCap : kLightS); i += kWave) {
     if constexpr (!PB) keys[i] = 0xffffffffu;
     vals[i] = V(0);
   }
@@ -3157,7 +3161,7 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
 #pragma unroll
           for (int u = 0; u < kLightU; ++u) {
             const uint32_t r = wr[u].y + __popc(wr[u].x & ((1u << (key[u] & 31)) - 1u));
-            if (ok[u] && r < (uint32_t)kLightS) lds_add(&vals[r], pv[u]);
+            if (ok[u] && r < (uint32_t)kLightCap) lds_add(&vals[r], pv[u]);
           }
           continue;
         }

@@ -906,7 +906,7 @@ static bool light_on(const ams_ctx_t* ctx, int W, int vbytes) {
 // row bitmaps, up to kLightMaxCnt (a table two-thirds full) in hash tables.
 static int light_limit(const ams_ctx_t* ctx, int W, int vbytes) {
   if (!light_on(ctx, W, vbytes)) return 0;
-  return std::min(ctx->prm.light_max, ctx->plan_pr.rowbits ? kLightS : kLightMaxCnt);
+  return std::min(ctx->prm.light_max, ctx->plan_pr.rowbits ? kLightCap : kLightMaxCnt);
 }
 
 template <typename V, bool PB>
@@ -1185,7 +1185,7 @@ void ams_default_params(ams_params_t* p) {
   // light morsels: at most 512 C entries at ranks from the kept row
   // bitmaps (341 in hash tables); G500 s19 ef32 dense route 243.2 ms at 341,
   // 237.7 ms at 512
-  p->light_max = kLightS;
+  p->light_max = kLightCap;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -1267,8 +1267,8 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (p->esc_sort < 0 || p->esc_sort > 4) return fail(ctx, AMS_ERR_ARG, "esc_sort must be in [0, 4]");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
-  if (p->light_max < 0 || p->light_max > kLightS)
-    return fail(ctx, AMS_ERR_ARG, "light_max must be in [0, %d]", kLightS);
+  if (p->light_max < 0 || p->light_max > std::max(kLightCap, kLightMaxCnt))
+    return fail(ctx, AMS_ERR_ARG, "light_max must be in [0, %d]", std::max(kLightCap, kLightMaxCnt));
   ctx->prm = *p;
   return AMS_OK;
 }
