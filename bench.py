@@ -372,7 +372,13 @@ def bench_workload(name, reps):
     """One-shot products of another workload (GFLOP/s = 2P/t), with the
     dominant kernel's roofline and the whole pipeline's fraction of the HBM
     roofline, and the all-ones checks (sum(C) = P, ascending columns)."""
+    import gc
+    import torch
     from ams_hip.device import Context, DeviceCsr
+    # earlier sections' tensors stay in torch's cache: return them to the
+    # device before this workload's C is allocated
+    gc.collect()
+    torch.cuda.empty_cache()
     t0 = time.perf_counter()
     a, b = make_inputs(name)
     gen_s = time.perf_counter() - t0

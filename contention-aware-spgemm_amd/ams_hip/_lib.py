@@ -27,7 +27,8 @@ class AmsParams(C.Structure):
                 ("acc_thres", C.c_double), ("dense_width", C.c_int32), ("bitmap_bits", C.c_int32),
                 ("bitmap_ratio", C.c_double), ("dense_density", C.c_double), ("mode", C.c_int32),
                 ("wgs_per_cu", C.c_int32), ("esc_sort", C.c_int32),
-                ("sym_blocked", C.c_int32), ("ghash_ratio", C.c_double), ("light_max", C.c_int32)]
+                ("sym_blocked", C.c_int32), ("ghash_ratio", C.c_double), ("light_max", C.c_int32),
+                ("row_bitmaps", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

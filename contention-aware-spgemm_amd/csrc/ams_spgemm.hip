@@ -665,7 +665,8 @@ static int run_symbolic(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b
   // dense chunks then skip the per-product bitmap test and OR) in
   // synchronised calls of the row-bitmap symbolic, when they take at most a
   // twelfth of the GPU's memory (G500 s19 ef32: 20 GB next to C's 246 GB).
-  if (ctx->plan_counts && !ctx->fixed && !ctx->prm.sym_blocked && H > 0 && ctx->mem_total > 0) {
+  if (ctx->prm.row_bitmaps && ctx->plan_counts && !ctx->fixed && !ctx->prm.sym_blocked && H > 0 &&
+      ctx->mem_total > 0) {
     const int64_t bwords = (std::max<int64_t>(pr.max_width, 1) + 31) / 32;
     const size_t bytes = (size_t)H * (size_t)bwords * 4;
     if (bytes <= ctx->mem_total / 12) {
@@ -1186,6 +1187,7 @@ void ams_default_params(ams_params_t* p) {
   // bitmaps (341 in hash tables); G500 s19 ef32 dense route 243.2 ms at 341,
   // 237.7 ms at 512
   p->light_max = kLightCap;
+  p->row_bitmaps = 1;
 }
 
 int ams_ctx_create(ams_ctx_t** out, void* stream) {
@@ -1267,6 +1269,7 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (p->esc_sort < 0 || p->esc_sort > 4) return fail(ctx, AMS_ERR_ARG, "esc_sort must be in [0, 4]");
   if (p->sym_blocked < 0 || p->sym_blocked > 1) return fail(ctx, AMS_ERR_ARG, "sym_blocked must be 0 or 1");
   if (!(p->ghash_ratio >= 0.0)) return fail(ctx, AMS_ERR_ARG, "ghash_ratio must be >= 0");
+  if (p->row_bitmaps < 0 || p->row_bitmaps > 1) return fail(ctx, AMS_ERR_ARG, "row_bitmaps must be 0 or 1");
   if (p->light_max < 0 || p->light_max > std::max(kLightCap, kLightMaxCnt))
     return fail(ctx, AMS_ERR_ARG, "light_max must be in [0, %d]", std::max(kLightCap, kLightMaxCnt));
   ctx->prm = *p;
@@ -1364,6 +1367,29 @@ int ams_get_error_flags(ams_ctx_t* ctx, int32_t* flags) {
   return AMS_OK;
 }
 
+// Releases the kept row bitmaps of the current plan; its numeric phase then
+// sets the dense chunks' bitmaps per product.
+static void drop_rowbits(ams_ctx_t* ctx) {
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->rowbits.p) (void)hipFree(ctx->rowbits.p);
+  (void)hipGetLastError();
+  ctx->rowbits = Buf{};
+  ++ctx->gen;
+  ctx->plan_pr.rowbits = nullptr;
+  ctx->plan_pr.bwords = 0;
+}
+
+// C's pool of nnz entries; when it does not fit beside the kept row
+// bitmaps, they are released and the pool is tried once more.
+static int grow_c_pool(ams_ctx_t* ctx, int64_t nnz, size_t vb) {
+  for (int attempt = 0;; ++attempt) {
+    int rc = grow(ctx, ctx->pool_col, sizeof(int32_t) * std::max<int64_t>(nnz, 1));
+    if (rc == AMS_OK) rc = grow(ctx, ctx->pool_val, vb * std::max<int64_t>(nnz, 1));
+    if (rc != AMS_ERR_ALLOC || attempt > 0 || !ctx->plan_pr.rowbits) return rc;
+    drop_rowbits(ctx);
+  }
+}
+
 int ams_spgemm(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dtype, ams_dcsr_t* c_out) {
   CHK(check_pair(ctx, a, b, dtype));
   if (!c_out) return fail(ctx, AMS_ERR_ARG, "c_out is NULL");
@@ -1372,8 +1398,7 @@ int ams_spgemm(ams_ctx_t* ctx, const ams_dcsr_t* a, const ams_dcsr_t* b, int dty
   int64_t nnz = 0;
   CHK(run_symbolic(ctx, a, b, dtype, single_probs(b), rp, true, &nnz));
   const size_t vb = dtype == AMS_F64 ? 8 : 4;
-  CHK(grow(ctx, ctx->pool_col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
-  CHK(grow(ctx, ctx->pool_val, vb * std::max<int64_t>(nnz, 1)));
+  CHK(grow_c_pool(ctx, nnz, vb));
   CHK(run_numeric(ctx, a, b, dtype, rp, (int32_t*)ctx->pool_col.p, ctx->pool_val.p, INT64_MAX));
   CHK(check_flags(ctx, "numeric"));
   c_out->num_row = a->num_row;

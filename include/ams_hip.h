@@ -77,6 +77,10 @@ typedef struct {
 	                           one wave each: at their columns' ranks when the symbolic
 	                           phase kept the row bitmaps (up to 512 entries), else in an
 	                           LDS hash table (up to 341); 0 = off, at most 512 */
+	int32_t row_bitmaps;    /* 1: synchronised symbolic calls keep the wide rows' column
+	                           bitmaps for the numeric phase (at most 1/12 of the GPU's
+	                           memory; ams_spgemm releases them when C would not fit
+	                           beside them); 0: off */
 } ams_params_t;
 
 /* Counters of the last symbolic / numeric call (rows per bin, tasks). */

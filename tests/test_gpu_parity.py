@@ -31,6 +31,14 @@ ROUTES = {
     # morsels of up to 8 C entries through the wave kernel, the rest through the block kernel
     "dense_light_mix": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=1024,
                             bitmap_bits=65536, dense_density=0.0, light_max=8),
+    # without the kept row bitmaps: light morsels in hash tables, the block kernel's bitmap per product
+    "all_dense_hash_light": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=128,
+                                 bitmap_bits=1024, dense_density=0.0, row_bitmaps=0),
+    "dense_mix_no_bitmaps": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=1024,
+                                 bitmap_bits=65536, dense_density=0.0, light_max=8, row_bitmaps=0),
+    # multi-pass bitmap symbolic (bitmap_bits below the row width) keeping its row bitmaps
+    "multipass_bitmaps_block": dict(sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=256,
+                                    bitmap_bits=512, dense_density=0.0, light_max=0),
     "bitmap_then_hash": dict(sym_hash_max=8, num_hash_max=5461, bitmap_ratio=0.0, dense_density=1.0,
                              bitmap_bits=256, dense_width=64),
     "big_tables": dict(sym_hash_max=5461, num_hash_max=5461, acc_thres=0.0),
@@ -81,7 +89,8 @@ def test_rmat_square(gpu_ctx, route, name, a, exact):
     assert_matches_oracle(c, a, a, exact)
 
 
-@pytest.mark.parametrize("route", ["default", "all_dense", "all_dense_block", "dense_light_mix", "esc_only",
+@pytest.mark.parametrize("route", ["default", "all_dense", "all_dense_block", "dense_light_mix",
+                                   "all_dense_hash_light", "dense_mix_no_bitmaps", "esc_only",
                                    "blocked_sym_dense", "esc_bucket", "esc_merge",
                                    "ghash"])
 def test_rectangular_unsorted(gpu_ctx, route):
