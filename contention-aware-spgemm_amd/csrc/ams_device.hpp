@@ -149,14 +149,15 @@ __device__ __forceinline__ uint32_t block_wave_prefix(const uint32_t* s, uint32_
 
 // Segment scan of a window of T segment lengths (one per thread, each >= 0):
 // off = products before this segment, ci = non-empty segments before it
-// (its compact index), total = products of the window (must fit 31 bits).
+// (its compact index), total = products of the window (must fit 31 bits),
+// nonempty = non-empty segments of the window.
 // A 32-bit DPP scan and a ballot per wave, then one barrier and a per-lane
 // combine of the waves' (products, non-empty) pairs in s_wave[T/64].  No
 // closing barrier: s_wave must not be written again before every thread has
 // passed another barrier.
 template <int T>
 __device__ __forceinline__ void block_seg_scan(int32_t len, uint2* s_wave, int32_t& off, int32_t& ci,
-                                               int32_t& total) {
+                                               int32_t& total, int32_t& nonempty) {
   const uint32_t incl = (uint32_t)wave_incl_scan(len);
   const unsigned long long ne = __ballot(len > 0);
   const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
@@ -166,6 +167,7 @@ __device__ __forceinline__ void block_seg_scan(int32_t len, uint2* s_wave, int32
   const uint32_t ia = wave_row_scan<T>(v.x), ib = wave_row_scan<T>(v.y);
   const int wv = __builtin_amdgcn_readfirstlane(wave_id());
   total = __builtin_amdgcn_readlane((int)ia, T / kWave - 1);
+  nonempty = __builtin_amdgcn_readlane((int)ib, T / kWave - 1);
   off = __builtin_amdgcn_readlane((int)(ia - v.x), wv) + (int32_t)incl - len;
   ci = __builtin_amdgcn_readlane((int)(ib - v.y), wv) + (int32_t)below;
 }

@@ -571,10 +571,8 @@ __global__ __launch_bounds__(T) void k_symb_hash(Csr A, Csr B, const int32_t* li
         len = (int32_t)(B.rp[k + 1] - st);
       }
       // the scan's barrier also orders the clearing above before the bits
-      int64_t tot64;
-      const int64_t ex = block_excl_scan_once<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
-      const int32_t o = (int32_t)(ex & 0xffffffffll), ci = (int32_t)(ex >> 32);
-      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
+      int32_t o, ci, total, nne;
+      block_seg_scan<T>(len, (uint2*)s_w, o, ci, total, nne);
       if (len > 0) {
         sbias[ci] = st - o;
         atomicOr(&wpre[o >> 5].x, 1u << (o & 31));
@@ -1727,10 +1725,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
       }
       // offsets and compact indices of the non-empty segments (the scan's
       // barrier also orders the bitmap clearing before the bits below)
-      int64_t tot64;
-      const int64_t ex = block_excl_scan_once<T, int64_t>(((int64_t)(len > 0) << 32) | (int64_t)len, s_w, tot64);
-      const int32_t o = (int32_t)(ex & 0xffffffffll), ci = (int32_t)(ex >> 32);
-      const int32_t total = (int32_t)(tot64 & 0xffffffffll);
+      int32_t o, ci, total, nne;
+      block_seg_scan<T>(len, (uint2*)s_w, o, ci, total, nne);
       if (w0 == p0) nmv = meta_load(nrow);  // the next row's bounds
       if (len > 0) {
         sb[ci] = SegBias<V>{st - o, a};
@@ -1739,8 +1735,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
       }
       if constexpr (MERGE) {
         if (threadIdx.x == 0) {
-          s_nrun = (int32_t)(tot64 >> 32);
-          roff[(int32_t)(tot64 >> 32)] = (uint16_t)total;
+          s_nrun = nne;
+          roff[nne] = (uint16_t)total;
         }
       }
       __syncthreads();
@@ -1868,8 +1864,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
       hd |= (uint32_t)head << r;
       heads += head;
     }
-    int64_t tot;
-    const int64_t rank0 = block_excl_scan<T, int64_t>(heads, s_w, tot);
+    int32_t tot;
+    const int32_t rank0 = block_excl_scan<T, int32_t>(heads, (int32_t*)s_w, tot);
     if (threadIdx.x == 0 && tot != z) atomicOr(err, 2);
     // Runs are summed in registers.  The run that ends at element r (the last
     // one before the next head, or the thread's last element when the run
@@ -2621,8 +2617,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
     for (int64_t w0 = d.p0; w0 < p1; w0 += T) {
       const int32_t len = (int32_t)(en - st);
       // scan of the segment lengths: offsets and compact index
-      int32_t off, ci, total;
-      block_seg_scan<T>(len, s_w[wc++ & 1], off, ci, total);
+      int32_t off, ci, total, nne;
+      block_seg_scan<T>(len, s_w[wc++ & 1], off, ci, total, nne);
       // prefetch the A entries of the next window: this row's next one, or
       // the next morsel's first; their B bounds follow after the passes
       const bool last = w0 + T >= p1;
