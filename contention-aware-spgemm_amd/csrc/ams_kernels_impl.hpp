@@ -1677,15 +1677,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
   // this thread's A entry of the row's first window and its B segment
   int32_t pk = 0;
   V pa = V(0);
-  int64_t pst = 0;
-  int32_t plen = 0;
+  // the next row's segment [pst, pen): its length is taken at use, so that
+  // the bounds' load is not waited for where it is issued
+  int64_t pst = 0, pen = 0;
   {
     const int64_t q0 = rd64(mv, 4) + threadIdx.x;
     if (q0 < rd64(mv, 6)) {
       pk = A.col[q0];
       pa = av[q0];
       pst = B.rp[pk];
-      plen = (int32_t)(B.rp[pk + 1] - pst);
+      pen = B.rp[pk + 1];
     }
   }
   for (; idx < cnt_rows; idx += gridDim.x) {
@@ -1709,7 +1710,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
     for (int64_t w0 = p0; w0 < p1; w0 += T) {
       for (int i = threadIdx.x; i < kWords; i += T) wpre[i] = make_uint2(0u, 0u);
       int64_t st = pst;
-      int32_t len = plen;
+      int32_t len = (int32_t)(pen - pst);
       V a = pa;
       if (w0 != p0) {  // windows after the first (rows of more than T A nonzeros)
         const int64_t p = w0 + tid;
@@ -1806,7 +1807,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
     pk = 0;
     pa = V(0);
     pst = 0;
-    plen = 0;
+    pen = 0;
     const int64_t q0 = rd64(nmv, 4) + tid;
     const bool pn = q0 < rd64(nmv, 6);
     if (pn) {
@@ -1814,7 +1815,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
       pa = av[q0];
     }
     __syncthreads();  // products in LDS; the region is free for the sort
-    const int i0 = threadIdx.x * IPT;
+    const int i0 = tid * IPT;  // from the row's opaque thread index: no hoisted address
     bool merged = false;
     if constexpr (MERGE) {
       if (p1 - p0 <= T) {  // one window: the runs are its B segments, keys go to their ranks
@@ -1841,7 +1842,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
     }
     if (pn) {  // and their B segments, while the runs are compressed
       pst = B.rp[pk];
-      plen = (int32_t)(B.rp[pk + 1] - pst);
+      pen = B.rp[pk + 1];
     }
     if (!merged) {
       if (in_lds) {  // this thread's IPT consecutive sorted keys
