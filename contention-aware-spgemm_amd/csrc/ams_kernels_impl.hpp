@@ -1785,11 +1785,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
           const int32_t seg = (int32_t)wp.y + __popc(wp.x & ((2u << (xq[j] & 31)) - 1u)) - 1;
           sg[j] = sb[seg < 0 ? 0 : seg];
         }
+        // unconditional loads (slots outside the window read B position 0,
+        // which exists when the row has products): straight-line code, so
+        // the first wait comes at the first use below, not after each load
 #pragma unroll
         for (int j = 0; j < G; ++j) {
           const int64_t q = in[j] ? sg[j].bias + xq[j] : 0;
-          cq[j] = in[j] ? B.col[q] : 0;
-          bq[j] = in[j] ? bv[q] : V(0);
+          cq[j] = B.col[q];
+          bq[j] = bv[q];
         }
 #pragma unroll
         for (int j = 0; j < G; ++j) {
