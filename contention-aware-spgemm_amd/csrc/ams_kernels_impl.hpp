@@ -845,7 +845,17 @@ __global__ __launch_bounds__(T) void k_heavy_finish(const Probs pr, const int32_
 // so every byte offset into A's columns and B's row pointers and columns
 // fits 32 bits and each load takes the scalar base plus a 32-bit vector
 // offset.
-template <int T, bool Q32, class Seg>
+// FULL: the pass covers the row's whole column range, so every product's
+// column is inside it (the clamp only keeps LDS addresses in bounds for
+// columns outside the problem), and lanes past the window's end, which
+// re-read its last product, may set that product's bit again: OR is
+// idempotent, so neither needs a mask.
+#ifndef AMS_SYMB_FULL_GROUP
+#define AMS_SYMB_FULL_GROUP 8
+#endif
+constexpr int kSymFullGroup = AMS_SYMB_FULL_GROUP;
+static_assert(kUSymBitmap % kSymFullGroup == 0, "whole groups");
+template <int T, bool Q32, bool FULL, class Seg>
 __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, ExpandLds<T>& L, const Seg& seg,
                                                    const Csr& B, uint32_t* bm, int64_t base, int32_t pw) {
   block_expand_batch<T, kUSymBitmap>(
@@ -859,6 +869,25 @@ __device__ __forceinline__ void symb_bitmap_expand(int64_t p0, int64_t p1, Expan
         }
       },
       [&](int32_t (&col)[kUSymBitmap], bool (&ok)[kUSymBitmap]) {
+        if constexpr (FULL) {
+          // groups of kSymFullGroup reads, then their ORs: the ORs order the
+          // next group's reads after them, which bounds the live registers
+#pragma unroll
+          for (int h = 0; h < kUSymBitmap; h += kSymFullGroup) {
+            uint32_t u[kSymFullGroup], w[kSymFullGroup];
+#pragma unroll
+            for (int j = 0; j < kSymFullGroup; ++j) {
+              u[j] = min((uint32_t)(col[h + j] - (int32_t)base), (uint32_t)pw - 1u);
+              w[j] = bm[u[j] >> 5];
+            }
+#pragma unroll
+            for (int j = 0; j < kSymFullGroup; ++j) {
+              const uint32_t bit = 1u << (u[j] & 31);
+              if (!(w[j] & bit)) atomicOr(&bm[u[j] >> 5], bit);
+            }
+          }
+          return;
+        }
         uint32_t u[kUSymBitmap], w[kUSymBitmap];
 #pragma unroll
         for (int j = 0; j < kUSymBitmap; ++j) {
@@ -890,7 +919,9 @@ constexpr int kSymGrab = 4;
 // base register), used by 1024-thread blocks, which two waves-limited
 // blocks per CU leave 80 KB of LDS each.
 constexpr int kSymStaticWords = 17152;
-template <int T, bool Q32, int SW>
+// FULL: Wb covers the widest problem, so every row takes one pass of its
+// whole column range (pw == width).
+template <int T, bool Q32, int SW, bool FULL>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE))) void k_symb_bitmap(Csr A, Csr B, Probs pr, const int64_t* row_flp,
                                                    const int32_t* list, const int32_t* counts, int bin,
                                                    int64_t* rownz, int32_t* coff, int R, int W, int Wb,
@@ -912,10 +943,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
   for (int32_t pi = blockIdx.x; pi < np; pi += gridDim.x) {
     const HeavyPiece hp = pieces[pi];
     if (hp.h < 0) continue;
-    const int32_t row = list[start + hp.h];
+    // uniform: in scalar registers, so that row, lo and width do not take
+    // vector registers (and spill) across the expansion
+    const int32_t row = __builtin_amdgcn_readfirstlane(list[start + hp.h]);
     int64_t lo, width;
     prob_cols(pr, row, lo, width);
-    const int64_t p0 = A.rp[row], deg = A.rp[row + 1] - p0;
+    lo = (int64_t)uniform_u64((unsigned long long)lo);
+    width = (int64_t)uniform_u64((unsigned long long)width);
+    const int64_t p0 = (int64_t)uniform_u64((unsigned long long)A.rp[row]);
+    const int64_t deg = (int64_t)uniform_u64((unsigned long long)A.rp[row + 1]) - p0;
     const int64_t q0 = p0 + deg * hp.i / hp.q, q1 = p0 + deg * (hp.i + 1) / hp.q;
     uint32_t* g = gbm + (int64_t)hp.hid * gwords;
     for (int64_t c0 = 0; c0 < width; c0 += Wb) {
@@ -924,7 +960,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       for (int w = threadIdx.x; w < words; w += T) bm[w] = 0u;
       __syncthreads();
       const int64_t base = lo + c0;
-      symb_bitmap_expand<T, Q32>(q0, q1, L, seg, B, bm, base, pw);
+      symb_bitmap_expand<T, Q32, FULL>(q0, q1, L, seg, B, bm, base, pw);
       // the thread index passes through an empty asm statement, so that the
       // compiler cannot hoist gbm + 4 * threadIdx.x out of the loops: held
       // across every row's product loop, that address spilled
@@ -958,12 +994,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       s_qtk = q_tk + 1;
     }
     __syncthreads();
-    const int32_t h = s_next;
+    const int32_t h = __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();  // every thread has read s_next before thread 0 writes it again
     if (h >= cnt) break;
-    const int32_t row = list[start + h];
+    // uniform values in scalar registers, as for the pieces above
+    const int32_t row = __builtin_amdgcn_readfirstlane(list[start + h]);
     int64_t lo, width;
     prob_cols(pr, row, lo, width);
+    lo = (int64_t)uniform_u64((unsigned long long)lo);
+    width = (int64_t)uniform_u64((unsigned long long)width);
     // k_symb_bitmap_wave's rows, when the host launches it (wave_rows)
     if (wave_rows && width <= kWaveBitmapBits && row_flp[row] <= kWaveBitmapFlops) continue;
     if (hflag && hflag[h]) continue;  // split into pieces above
@@ -977,10 +1016,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
-      symb_bitmap_expand<T, Q32>(A.rp[row], A.rp[row + 1], L, seg, B, bm, base, pw);
+      symb_bitmap_expand<T, Q32, FULL>((int64_t)uniform_u64((unsigned long long)A.rp[row]),
+                                       (int64_t)uniform_u64((unsigned long long)A.rp[row + 1]), L, seg, B, bm, base,
+                                       pw);
       if (pr.rowbits) {  // the row's column bitmap, for the numeric phase
         uint32_t* rb = pr.rowbits + (int64_t)h * pr.bwords + (c0 >> 5);
-        for (int w = threadIdx.x; w < words; w += T) rb[w] = bm[w];
+        int tid = (int)threadIdx.x;  // opaque, as above: no hoisted rowbits + 4 * threadIdx.x
+        asm volatile("" : "+v"(tid));
+        for (int w = tid; w < words; w += T) rb[w] = bm[w];
       }
       // words_pass rounded up to whole waves so every lane joins the vote
       for (int w0 = 0; w0 < words; w0 += T) {

@@ -488,22 +488,22 @@ static void launch_symb_hash(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t
 
 // The bitmap bin's wide rows (and heavy-row pieces, when hflag is set), one
 // row per T-thread block with dyn bytes of bitmap and chunk counters.
-template <int T, bool Q32, int SW>
+template <int T, bool Q32, int SW, bool FULL = false>
 static int launch_symb_bitmap_q(ams_ctx_t* ctx, const Csr& A, const Csr& B, const Probs& pr, int64_t H, size_t dyn,
                                 int64_t* rownz, int R, int W, int Wb, const uint8_t* hflag, int64_t gwords,
                                 int wave_rows) {
   if (SW > 0) dyn = 0;  // the bitmap is the kernel's static array
   if (dyn + sizeof(ExpandLds<T>) + (size_t)SW * 4 > ctx->lds_max)
     return fail(ctx, AMS_ERR_ARG, "bitmap_bits %d needs %zu B of LDS", Wb, dyn);
-  HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<T, Q32, SW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)dyn));
+  HIPCHK(hipFuncSetAttribute((const void*)k_symb_bitmap<T, Q32, SW, FULL>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
   Ctrl* c = dctrl(ctx);
   int occ;
-  occupancy(k_symb_bitmap<T, Q32, SW>, T, dyn, &occ);
+  occupancy(k_symb_bitmap<T, Q32, SW, FULL>, T, dyn, &occ);
   const int g = grid_for(ctx, occ, H, ctx->plan_counts);
   // c->sym_ticket starts at zero: the control block is cleared at the start
   // of every symbolic phase, which launches this kernel once
-  hipLaunchKernelGGL((k_symb_bitmap<T, Q32, SW>), dim3(g), dim3(T), dyn, ctx->stream, A, B, pr,
+  hipLaunchKernelGGL((k_symb_bitmap<T, Q32, SW, FULL>), dim3(g), dim3(T), dyn, ctx->stream, A, B, pr,
                      (const int64_t*)ctx->row_flp.p,
                      (const int32_t*)ctx->lsym.p, (const int32_t*)c->cnt_sym, (int)SB_BITMAP, rownz,
                      (int32_t*)ctx->coff.p, R, W, Wb, hflag, (const HeavyPiece*)ctx->hpiece.p,
@@ -517,6 +517,12 @@ static int launch_symb_bitmap(ams_ctx_t* ctx, const Csr& A, const Csr& B, const 
                               int wave_rows) {
   const bool q32 = A.nnz < ((int64_t)1 << 30) && B.nnz < ((int64_t)1 << 30) && B.m < ((int64_t)1 << 28);
   if constexpr (T == 1024) {
+    // every row in one pass of its whole width: no per-product range test
+    if ((int64_t)(Wb >> 5) + Wb / W + 1 <= kSymStaticWords && Wb >= pr.max_width)
+      return q32 ? launch_symb_bitmap_q<T, true, kSymStaticWords, true>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag,
+                                                                        gwords, wave_rows)
+                 : launch_symb_bitmap_q<T, false, kSymStaticWords, true>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb,
+                                                                         hflag, gwords, wave_rows);
     if ((int64_t)(Wb >> 5) + Wb / W + 1 <= kSymStaticWords)
       return q32 ? launch_symb_bitmap_q<T, true, kSymStaticWords>(ctx, A, B, pr, H, dyn, rownz, R, W, Wb, hflag, gwords,
                                                                   wave_rows)
