@@ -1019,23 +1019,69 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       symb_bitmap_expand<T, Q32, FULL>((int64_t)uniform_u64((unsigned long long)A.rp[row]),
                                        (int64_t)uniform_u64((unsigned long long)A.rp[row + 1]), L, seg, B, bm, base,
                                        pw);
-      if (pr.rowbits) {  // the row's column bitmap, for the numeric phase
-        uint32_t* rb = pr.rowbits + (int64_t)h * pr.bwords + (c0 >> 5);
-        int tid = (int)threadIdx.x;  // opaque, as above: no hoisted rowbits + 4 * threadIdx.x
-        asm volatile("" : "+v"(tid));
-        for (int w = tid; w < words; w += T) rb[w] = bm[w];
-      }
-      // words_pass rounded up to whole waves so every lane joins the vote
-      for (int w0 = 0; w0 < words; w0 += T) {
-        const int w = w0 + threadIdx.x;
-        const int pc = w < words ? __popc(bm[w]) : 0;
-        const int cidx = w < words ? w / wchunk : -1;
-        const int c_lane0 = __shfl(cidx, 0, kWave);
-        if (__all(cidx == c_lane0)) {
-          const int tot = wave_sum(pc);
-          if (lane_id() == 0 && tot && c_lane0 >= 0) atomicAdd(&ccount[c_lane0], tot);
-        } else if (pc) {
-          atomicAdd(&ccount[cidx], pc);
+      // the row's column bitmap (for the numeric phase) and the populations
+      // of its chunks
+      uint32_t* rb = pr.rowbits ? pr.rowbits + (int64_t)h * pr.bwords + (c0 >> 5) : nullptr;
+      if (wchunk >= kWave) {
+        // wave v takes the words [v*per, (v+1)*per) (per a multiple of 64) in
+        // coalesced 64-word steps, stores them to rb and sums their
+        // populations per lane; the sum goes to the chunk's counter (one wave
+        // sum, one LDS add) only when the range leaves the chunk, and a step of
+        // 64 <= wchunk words crosses at most one chunk boundary
+        constexpr int kNw = T / kWave;
+        const int per = ((words + kNw - 1) / kNw + kWave - 1) / kWave * kWave;
+        const int k0 = __builtin_amdgcn_readfirstlane(wave_id()) * per;
+        const int k1 = k0 + per < words ? k0 + per : words;
+        int cur = k0 / wchunk;
+        int cend = (cur + 1) * wchunk;  // first word of the next chunk
+        int acc = 0;
+        auto flush = [&](int c, int v) {
+          const int tot = wave_sum(v);
+          if (lane_id() == 0 && tot) atomicAdd(&ccount[c], tot);
+        };
+        for (int k = k0; k < k1; k += kWave) {
+          if (k >= cend) {  // the step starts the next chunk
+            flush(cur, acc);
+            acc = 0;
+            ++cur;
+            cend += wchunk;
+          }
+          const int w = k + lane_id();
+          uint32_t x = 0u;
+          if (w < k1) {
+            x = bm[w];
+            if (rb) rb[w] = x;
+          }
+          const int pc = __popc(x);
+          if (k + kWave > cend) {  // the step's words from cend on are the next chunk's
+            const int nx = w >= cend ? pc : 0;
+            flush(cur, acc + pc - nx);
+            acc = nx;
+            ++cur;
+            cend += wchunk;
+          } else {
+            acc += pc;
+          }
+        }
+        if (k0 < k1) flush(cur, acc);
+      } else {
+        if (rb) {
+          int tid = (int)threadIdx.x;  // opaque, as above: no hoisted rowbits + 4 * threadIdx.x
+          asm volatile("" : "+v"(tid));
+          for (int w = tid; w < words; w += T) rb[w] = bm[w];
+        }
+        // words_pass rounded up to whole waves so every lane joins the vote
+        for (int w0 = 0; w0 < words; w0 += T) {
+          const int w = w0 + threadIdx.x;
+          const int pc = w < words ? __popc(bm[w]) : 0;
+          const int cidx = w < words ? w / wchunk : -1;
+          const int c_lane0 = __shfl(cidx, 0, kWave);
+          if (__all(cidx == c_lane0)) {
+            const int tot = wave_sum(pc);
+            if (lane_id() == 0 && tot && c_lane0 >= 0) atomicAdd(&ccount[c_lane0], tot);
+          } else if (pc) {
+            atomicAdd(&ccount[cidx], pc);
+          }
         }
       }
       __syncthreads();
