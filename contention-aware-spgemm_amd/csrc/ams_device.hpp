@@ -249,9 +249,10 @@ __device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, Expan
     const int32_t total = (int32_t)total64;
     const int nvalid = (int)((p1 - w0) < NW ? (p1 - w0) : NW);
     // the lane's current segment stays in registers: [.., seg_end) and
-    // B position = seg_bias + t; LDS is searched only when t leaves it
-    int cur = 0;
-    int32_t seg_end = -1;
+    // B position = seg_bias + t; LDS is searched only when t leaves it.
+    // Before the first product: segment -1, which ends at off[0] = 0
+    int cur = -1;
+    int32_t seg_end = 0;
     int64_t seg_bias = 0;
     for (int32_t t0 = 0; t0 < total; t0 += T * U) {
       Item item[U];
@@ -264,9 +265,21 @@ __device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, Expan
         ok[u] = t < total;
         t = ok[u] ? t : total - 1;
         if (t >= seg_end) {
-          cur = seek_segment(L.off, nvalid, cur, t);
-          seg_end = cur + 1 < nvalid ? L.off[cur + 1] : total;
-          seg_bias = L.start[cur] - (int64_t)L.off[cur];
+          // mostly the next segment, which starts at seg_end: its end and
+          // start position are two independent reads; past it (an empty or
+          // short segment), gallop from the one after, whose offset <= t
+          const int nx = cur + 1;
+          const int32_t nend = nx + 1 < nvalid ? L.off[nx + 1] : total;
+          const int64_t nst = L.start[nx];
+          if (t < nend) {
+            cur = nx;
+            seg_bias = nst - (int64_t)seg_end;
+            seg_end = nend;
+          } else {
+            cur = seek_segment(L.off, nvalid, nx + 1, t);
+            seg_end = cur + 1 < nvalid ? L.off[cur + 1] : total;
+            seg_bias = L.start[cur] - (int64_t)L.off[cur];
+          }
         }
         item[u] = load(cur, w0 + cur, seg_bias + t, base + t);
       }
