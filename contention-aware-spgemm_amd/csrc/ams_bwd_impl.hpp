@@ -83,8 +83,8 @@ __device__ __forceinline__ void block_expand_flush(int64_t p0, int64_t p1, Expan
     __syncthreads();
     const int32_t total = (int32_t)total64;
     const int nvalid = (int)((p1 - w0) < T ? (p1 - w0) : T);
-    int cur = 0;
-    int32_t seg_end = -1;
+    int cur = -1;  // segment_advance's start: segment -1 ends at off[0] = 0
+    int32_t seg_end = 0;
     int64_t seg_bias = 0;
     for (int32_t t0 = 0; t0 < total; t0 += T * U) {
       bool ok[U];
@@ -95,11 +95,7 @@ __device__ __forceinline__ void block_expand_flush(int64_t p0, int64_t p1, Expan
         int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
         ok[u] = t < total;
         t = ok[u] ? t : total - 1;
-        if (t >= seg_end) {
-          cur = seek_segment(L.off, nvalid, cur, t);
-          seg_end = cur + 1 < nvalid ? L.off[cur + 1] : total;
-          seg_bias = L.start[cur] - (int64_t)L.off[cur];
-        }
+        if (t >= seg_end) segment_advance(L.off, L.start, nvalid, total, t, cur, seg_end, seg_bias);
         j[u] = cur;
         q[u] = seg_bias + t;
       }

@@ -207,6 +207,29 @@ __device__ __forceinline__ int seek_segment(const int32_t* off, int n, int cur, 
   return lo;
 }
 
+// Moves a lane's segment cursor forward to the segment holding product t
+// (t >= seg_end, t < total) of a window of n segments with exclusive offsets
+// off[] and first B positions start[]; afterwards B position = seg_bias + t
+// for t < seg_end.  Start with cur = -1, seg_end = 0 (off[0] = 0).  Mostly
+// the next segment, which starts at seg_end: its end and start position are
+// two independent reads; past it (an empty or short segment), gallop from
+// the one after, whose offset is <= t.
+__device__ __forceinline__ void segment_advance(const int32_t* off, const int64_t* start, int n, int32_t total,
+                                                int32_t t, int& cur, int32_t& seg_end, int64_t& seg_bias) {
+  const int nx = cur + 1;
+  const int32_t nend = nx + 1 < n ? off[nx + 1] : total;
+  const int64_t nst = start[nx];
+  if (t < nend) {
+    cur = nx;
+    seg_bias = nst - (int64_t)seg_end;
+    seg_end = nend;
+  } else {
+    cur = seek_segment(off, n, nx + 1, t);
+    seg_end = cur + 1 < n ? off[cur + 1] : total;
+    seg_bias = start[cur] - (int64_t)off[cur];
+  }
+}
+
 // Enumerates the products of A nonzeros [p0, p1): nonzero p contributes the
 // B positions [start(p), start(p) + len(p)).  Windows of NW nonzeros (the
 // block size T unless the kernel saves LDS with fewer) are flattened with a
@@ -264,23 +287,7 @@ __device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, Expan
         int32_t t = t0 + wave_id() * (kWave * U) + u * kWave + lane_id();
         ok[u] = t < total;
         t = ok[u] ? t : total - 1;
-        if (t >= seg_end) {
-          // mostly the next segment, which starts at seg_end: its end and
-          // start position are two independent reads; past it (an empty or
-          // short segment), gallop from the one after, whose offset <= t
-          const int nx = cur + 1;
-          const int32_t nend = nx + 1 < nvalid ? L.off[nx + 1] : total;
-          const int64_t nst = L.start[nx];
-          if (t < nend) {
-            cur = nx;
-            seg_bias = nst - (int64_t)seg_end;
-            seg_end = nend;
-          } else {
-            cur = seek_segment(L.off, nvalid, nx + 1, t);
-            seg_end = cur + 1 < nvalid ? L.off[cur + 1] : total;
-            seg_bias = L.start[cur] - (int64_t)L.off[cur];
-          }
-        }
+        if (t >= seg_end) segment_advance(L.off, L.start, nvalid, total, t, cur, seg_end, seg_bias);
         item[u] = load(cur, w0 + cur, seg_bias + t, base + t);
       }
       apply_batch(item, ok);
@@ -339,8 +346,8 @@ __device__ __forceinline__ void wave_expand(int64_t p0, int64_t p1, int32_t* w_o
     w_start[lane] = st;
     wave_lds_sync();
     const int nvalid = (int)((p1 - w0) < kWave ? (p1 - w0) : kWave);
-    int cur = 0;
-    int32_t seg_end = -1;
+    int cur = -1;
+    int32_t seg_end = 0;
     int64_t seg_bias = 0;
     for (int32_t t0 = 0; t0 < total; t0 += kWave * U) {
       Item item[U];
@@ -350,11 +357,7 @@ __device__ __forceinline__ void wave_expand(int64_t p0, int64_t p1, int32_t* w_o
         int32_t t = t0 + u * kWave + lane;
         ok[u] = t < total;
         t = ok[u] ? t : total - 1;
-        if (t >= seg_end) {
-          cur = seek_segment(w_off, nvalid, cur, t);
-          seg_end = cur + 1 < nvalid ? w_off[cur + 1] : total;
-          seg_bias = w_start[cur] - (int64_t)w_off[cur];
-        }
+        if (t >= seg_end) segment_advance(w_off, w_start, nvalid, total, t, cur, seg_end, seg_bias);
         item[u] = load(cur, w0 + cur, seg_bias + t, base + t);
       }
 #pragma unroll
