@@ -3425,11 +3425,18 @@ __global__ __launch_bounds__(256) void k_route_sums(int64_t m, const int64_t* ro
       }
     }
   }
+  // waves add into the block's LDS sums, then one global add a sum a block:
+  // every block's adds go to the same two cache lines, where they serialise
+  __shared__ unsigned long long s_sum[kRouteSums];
+  if (threadIdx.x < kRouteSums) s_sum[threadIdx.x] = 0ull;
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < kRouteSums; ++k) {
     const int64_t v = wave_sum(acc[k]);
-    if (lane_id() == 0 && v) atomicAdd(&sums[k], (unsigned long long)v);
+    if (lane_id() == 0 && v) atomicAdd(&s_sum[k], (unsigned long long)v);
   }
+  __syncthreads();
+  if (threadIdx.x < kRouteSums && s_sum[threadIdx.x]) atomicAdd(&sums[threadIdx.x], s_sum[threadIdx.x]);
 }
 
 // Checksums of C: out[0] += sum of columns, out[1] += rows whose columns

@@ -411,10 +411,13 @@ static int dense_block(int W, int vbytes) {
 
 // Row flops: a wave per row, or the nonzero split when rows are long on
 // average (a wave per row would leave most of the GPU idle).
+#ifndef AMS_FLP_NZ_AVG
+#define AMS_FLP_NZ_AVG 256
+#endif
 static int launch_row_flops(ams_ctx_t* ctx, const Csr& A, const Csr& B, int64_t* row_flp,
                             unsigned long long* total) {
   if (A.m == 0) return AMS_OK;
-  if (row_flp && A.nnz > 256 * A.m) {
+  if (row_flp && A.nnz > (int64_t)AMS_FLP_NZ_AVG * A.m) {
     CHK(fill_i64(ctx, row_flp, A.m, 0));
     const int64_t threads = (A.nnz + kFlpIpt - 1) / kFlpIpt;
     const int g = (int)std::min<int64_t>((threads + 255) / 256, (int64_t)ctx->num_cu * 32);
@@ -551,8 +554,9 @@ static int route_numeric(ams_ctx_t* ctx, const Csr& A, const Probs& pr, const in
                        np, (uint8_t*)ctx->nbin.p, c->cnt_num, &c->ghash_num_bytes);
     hipLaunchKernelGGL(k_scatter_bins, dim3(gc), dim3(256), 0, ctx->stream, m, (const uint8_t*)ctx->nbin.p,
                        kNumBins, (const int32_t*)c->cnt_num, c->cur_num, (int32_t*)ctx->lnum.p);
-    if (sync)
-      hipLaunchKernelGGL(k_route_sums, dim3(gc), dim3(256), 0, ctx->stream, m, (const int64_t*)ctx->row_flp.p,
+    if (sync)  // one block a CU: its sums go to global memory once a block
+      hipLaunchKernelGGL(k_route_sums, dim3((unsigned)std::min<int64_t>(gc, ctx->num_cu)), dim3(256), 0, ctx->stream, m,
+                         (const int64_t*)ctx->row_flp.p,
                          (const int64_t*)rp_c, A.rp, (const uint8_t*)ctx->sbin.p, (const uint8_t*)ctx->nbin.p,
                          c->sums);
     HIPCHK(hipGetLastError());
