@@ -210,7 +210,10 @@ __global__ __launch_bounds__(256) void k_row_flops(Csr A, Csr B, int64_t* row_fl
 // stride's A columns, then its B bounds, are all loaded before any is used,
 // so a hub row of 10^5 nonzeros (which sets the kernel's time) takes a few
 // hundred load latencies instead of a few thousand.
-constexpr int kFlpUnroll = 8;
+#ifndef AMS_FLP_UNROLL
+#define AMS_FLP_UNROLL 8
+#endif
+constexpr int kFlpUnroll = AMS_FLP_UNROLL;
 __global__ __launch_bounds__(256) void k_row_flops_wave(Csr A, Csr B, int64_t* row_flp,
                                                         unsigned long long* total) {
   __shared__ int64_t s_w[4];
@@ -428,7 +431,10 @@ __global__ __launch_bounds__(256) void k_classify_num(int64_t m, const int64_t* 
         } else if (np.mode != 1 && f <= np.esc_max &&
                    (np.mode == 2 || (double)f / (double)z < np.acc_thres)) {
           b = kNumHashBins + esc_bin_of(f);
-        } else if (z <= np.num_hash_max) {
+        } else if (z <= np.num_hash_max || sbin[i] < SB_BITMAP) {
+          // LDS-hash symbolic rows have no chunk counts for the dense route;
+          // their z <= f <= sym_hash_max fits an LDS table whatever
+          // num_hash_max is
           const int64_t s = next_pow2_i64(z + ((z + 1) >> 1));
           b = s <= 64 ? 0 : ilog2_pow2(s) - 6;
         } else if (sbin[i] == SB_GHASH) {

@@ -1269,8 +1269,8 @@ int ams_set_params(ams_ctx_t* ctx, const ams_params_t* p) {
   if (!ctx || !p) return AMS_ERR_ARG;
   if (p->sym_hash_max < 1 || p->sym_hash_max > (kMaxHashSlots * 2) / 3)
     return fail(ctx, AMS_ERR_ARG, "sym_hash_max must be in [1, %d]", (kMaxHashSlots * 2) / 3);
-  if (p->num_hash_max < p->sym_hash_max || p->num_hash_max > (kMaxHashSlots * 2) / 3)
-    return fail(ctx, AMS_ERR_ARG, "num_hash_max must be in [sym_hash_max, %d]", (kMaxHashSlots * 2) / 3);
+  if (p->num_hash_max < 1 || p->num_hash_max > (kMaxHashSlots * 2) / 3)
+    return fail(ctx, AMS_ERR_ARG, "num_hash_max must be in [1, %d]", (kMaxHashSlots * 2) / 3);
   if (p->esc_max < 0 || p->esc_max > kMaxEsc) return fail(ctx, AMS_ERR_ARG, "esc_max must be in [0, %d]", kMaxEsc);
   if (p->dense_width < 64 || p->dense_width > 65536 || p->dense_width % 64)
     return fail(ctx, AMS_ERR_ARG, "dense_width must be a multiple of 64 in [64, 65536]");

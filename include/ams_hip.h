@@ -52,7 +52,9 @@ typedef struct {
  * knobs (cmake/Options.cmake:104-191); ams_autotune() picks them. */
 typedef struct {
 	int32_t sym_hash_max;   /* rows with flops <= this use the LDS-hash symbolic      */
-	int32_t num_hash_max;   /* rows with nnz(C row) <= this may use an LDS accumulator */
+	int32_t num_hash_max;   /* bitmap-symbolic rows with nnz(C row) <= this may use an
+	                           LDS accumulator, larger ones take the dense chunks;
+	                           LDS-hash-symbolic rows always fit one                 */
 	int32_t esc_max;        /* rows with flops <= this may use ESC                   */
 	double  acc_thres;      /* flops / nnz(C row) below this -> ESC (AMS_ACC_THRES)  */
 	int32_t dense_width;    /* columns per dense LDS chunk (multiple of 64) for FP64;

@@ -59,4 +59,4 @@ def test_params_struct_layout_matches_c():
     p = _lib.AmsParams()
     L.ams_default_params(ctypes.byref(p))
     assert p.mode == 0 and p.acc_thres == 2.0 and p.dense_width % 64 == 0
-    assert p.sym_hash_max <= p.num_hash_max
+    assert 1 <= p.sym_hash_max <= 5461 and 1 <= p.num_hash_max <= 5461

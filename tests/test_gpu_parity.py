@@ -42,6 +42,9 @@ ROUTES = {
     "bitmap_then_hash": dict(sym_hash_max=8, num_hash_max=5461, bitmap_ratio=0.0, dense_density=1.0,
                              bitmap_bits=256, dense_width=64),
     "big_tables": dict(sym_hash_max=5461, num_hash_max=5461, acc_thres=0.0),
+    # num_hash_max below sym_hash_max: LDS-hash-symbolic rows above it still take LDS
+    # tables, bitmap-symbolic rows above it the dense chunks (no ESC)
+    "hash_sym_above_num": dict(sym_hash_max=2048, num_hash_max=16, acc_thres=0.0),
     "blocked_sym": dict(sym_blocked=1, sym_hash_max=64, num_hash_max=64, bitmap_ratio=0.0),
     "blocked_sym_dense": dict(sym_blocked=1, sym_hash_max=1, num_hash_max=1, bitmap_ratio=0.0, dense_width=128,
                               bitmap_bits=1024, dense_density=0.0),
