@@ -262,14 +262,15 @@ __device__ __forceinline__ void block_expand_batch(int64_t p0, int64_t p1, Expan
     int64_t st = 0;
     int32_t len = 0;
     if (loader && p < p1) seg(p, st, len);
-    int64_t total64;
-    const int64_t ex = block_excl_scan<T, int64_t>((int64_t)len, L.wtmp, total64);
+    // 32-bit segment scan (a window's products fit 31 bits): one barrier;
+    // L.wtmp is next written after this window's closing barrier
+    int32_t ex, ci, total, nne;
+    block_seg_scan<T>(len, (uint2*)L.wtmp, ex, ci, total, nne);
     if (loader) {
-      L.off[threadIdx.x] = (int32_t)ex;
+      L.off[threadIdx.x] = ex;
       L.start[threadIdx.x] = st;
     }
     __syncthreads();
-    const int32_t total = (int32_t)total64;
     const int nvalid = (int)((p1 - w0) < NW ? (p1 - w0) : NW);
     // the lane's current segment stays in registers: [.., seg_end) and
     // B position = seg_bias + t; LDS is searched only when t leaves it.

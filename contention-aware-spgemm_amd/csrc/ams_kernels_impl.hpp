@@ -943,6 +943,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
   bin_range(counts, bin, start, cnt);
   const std::conditional_t<Q32, SegB32, SegB> seg{A.col, B.rp};
   const int wchunk = W >> 5;
+  // the bitmap is cleared once here; afterwards each pass's reader (the
+  // pieces' OR into the global bitmap, the rows' chunk count) zeroes every
+  // word it reads, and those are the only words a pass can set
+  for (int w = threadIdx.x; w < (Wb >> 5); w += T) bm[w] = 0u;
+  __syncthreads();
   // the pieces of heavy rows first (they are the longest work items): each
   // ORs its bitmap into the row's global bitmap, word by non-zero word
   const int32_t np = hflag ? (*npieces < kMaxPieces ? *npieces : kMaxPieces) : 0;
@@ -963,8 +968,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
     for (int64_t c0 = 0; c0 < width; c0 += Wb) {
       const int32_t pw = (int32_t)((width - c0) < Wb ? (width - c0) : Wb);
       const int32_t words = (pw + 31) >> 5;
-      for (int w = threadIdx.x; w < words; w += T) bm[w] = 0u;
-      __syncthreads();
       const int64_t base = lo + c0;
       symb_bitmap_expand<T, Q32, FULL>(q0, q1, L, seg, B, bm, base, pw);
       // the thread index passes through an empty asm statement, so that the
@@ -974,6 +977,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       asm volatile("" : "+v"(tid));
       for (int w = tid; w < words; w += T) {
         const uint32_t x = bm[w];
+        bm[w] = 0u;
         if (x) atomicOr(&g[(c0 >> 5) + w], x);
       }
       __syncthreads();
@@ -1018,7 +1022,6 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
       const int32_t pw = (int32_t)((width - c0) < Wb ? (width - c0) : Wb);
       const int32_t words = (pw + 31) >> 5;
       const int32_t nch = (pw + W - 1) / W;
-      for (int w = threadIdx.x; w < words; w += T) bm[w] = 0u;
       for (int c = threadIdx.x; c < nch; c += T) ccount[c] = 0;
       __syncthreads();
       const int64_t base = lo + c0;
@@ -1056,6 +1059,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
           uint32_t x = 0u;
           if (w < k1) {
             x = bm[w];
+            bm[w] = 0u;
             if (rb) rb[w] = x;
           }
           const int pc = __popc(x);
@@ -1079,7 +1083,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
         // words_pass rounded up to whole waves so every lane joins the vote
         for (int w0 = 0; w0 < words; w0 += T) {
           const int w = w0 + threadIdx.x;
-          const int pc = w < words ? __popc(bm[w]) : 0;
+          uint32_t x = 0u;
+          if (w < words) {
+            x = bm[w];
+            bm[w] = 0u;
+          }
+          const int pc = __popc(x);
           const int cidx = w < words ? w / wchunk : -1;
           const int c_lane0 = __shfl(cidx, 0, kWave);
           if (__all(cidx == c_lane0)) {
