@@ -25,6 +25,27 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// Store of write-once output (the dense chunks' C entries, the kept row
+// bitmaps): with AMS_NT_STORES set it is a non-temporal store, which streams
+// past the caches instead of evicting the B blocks and columns the kernels
+// re-read.  Only for runs of consecutive entries: the light kernel's
+// scattered per-bit column stores took 20 -> 57 ms on G500 s19 ef32 with it.
+#ifndef AMS_NT_STORES
+#define AMS_NT_STORES 1
+#endif
+template <class T>
+struct same_type {
+  using type = T;
+};
+template <class T>
+__device__ __forceinline__ void stream_store(T* p, typename same_type<T>::type v) {
+#if AMS_NT_STORES
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // DPP lane move of a wave64 (GFX9 controls): row_shr:n reads lane i - n of
 // the same 16-lane row, row_bcast:15 / row_bcast:31 read the last lane of
 // the previous row / of row 1 into the rows of kRowMask; lanes without a

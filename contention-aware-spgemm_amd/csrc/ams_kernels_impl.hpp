@@ -723,7 +723,7 @@ __global__ __launch_bounds__(T) void k_symb_bitmap_wave(Csr A, Csr B, Probs pr, 
     wave_lds_sync();
     if (pr.rowbits) {
       uint32_t* rb = pr.rowbits + (int64_t)h * pr.bwords;
-      for (int w = lane_id(); w < words; w += kWave) rb[w] = bm[w];
+      for (int w = lane_id(); w < words; w += kWave) stream_store(&rb[w], bm[w]);
     }
     int32_t* crow = coff + (int64_t)h * (R + 1);
     const int nch = (int)((width + W - 1) / W);
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(T) void k_heavy_finish(const Probs pr, const int32_
       for (int64_t w = w0 + threadIdx.x; w < w1; w += T) {
         const uint32_t x = g[w];
         pc += __popc(x);
-        if (rb) rb[w] = x;
+        if (rb) stream_store(&rb[w], x);
         g[w] = 0u;
       }
       const int64_t tot = block_sum<T, int64_t>(pc, s_w);
@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
           if (w < k1) {
             x = bm[w];
             bm[w] = 0u;
-            if (rb) rb[w] = x;
+            if (rb) stream_store(&rb[w], x);
           }
           const int pc = __popc(x);
           if (k + kWave > cend) {  // the step's words from cend on are the next chunk's
@@ -1078,7 +1078,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(AMS_SYMB_WPE)
         if (rb) {
           int tid = (int)threadIdx.x;  // opaque, as above: no hoisted rowbits + 4 * threadIdx.x
           asm volatile("" : "+v"(tid));
-          for (int w = tid; w < words; w += T) rb[w] = bm[w];
+          for (int w = tid; w < words; w += T) stream_store(&rb[w], bm[w]);
         }
         // words_pass rounded up to whole waves so every lane joins the vote
         for (int w0 = 0; w0 < words; w0 += T) {
@@ -2969,8 +2969,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
           V v = V(0);
           if (valid) v = lds_take(&dense[u]);
           if (valid && r < d.cnt) {
-            colp[r] = d.colbase + u;
-            valp[r] = v;
+            stream_store(&colp[r], d.colbase + u);
+            stream_store(&valp[r], v);
           }
         }
         wave_lds_sync();  // the staging row is read before the next group writes it
@@ -2991,12 +2991,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= 1024 ? A
         if (seta) va = lds_take(&dense[ua]);
         if (setb) vb = lds_take(&dense[ub]);
         if (seta && ra < d.cnt) {
-          colp[ra] = d.colbase + ua;
-          valp[ra] = va;
+          stream_store(&colp[ra], d.colbase + ua);
+          stream_store(&valp[ra], va);
         }
         if (setb && rb < d.cnt) {
-          colp[rb] = d.colbase + ub;
-          valp[rb] = vb;
+          stream_store(&colp[rb], d.colbase + ub);
+          stream_store(&valp[rb], vb);
         }
       }
     }
