@@ -25,8 +25,9 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
   return ((unsigned long long)hi << 32) | lo;
 }
 
-// Store of write-once output (the dense chunks' C entries, the kept row
-// bitmaps): with AMS_NT_STORES set it is a non-temporal store, which streams
+// Store of write-once output (C entries stored in coalesced runs by the
+// dense chunks, ESC and the LDS-hash numeric; the kept row bitmaps): with
+// AMS_NT_STORES set it is a non-temporal store, which streams
 // past the caches instead of evicting the B blocks and columns the kernels
 // re-read.  Only for runs of consecutive entries: the light kernel's
 // scattered per-bit column stores took 20 -> 57 ms on G500 s19 ef32 with it.

@@ -1253,8 +1253,8 @@ __global__ __launch_bounds__(T) void k_numc_hash(Csr A, Csr B, const int32_t* li
     for (int r = threadIdx.x; r < z; r += T) {
       const int32_t key = sk[r];
       const uint32_t slot = lds_find(keys, shift, key);
-      col_c[off + r] = key;
-      val_c[off + r] = slot < (uint32_t)S ? vals[slot] : V(0);
+      stream_store(&col_c[off + r], key);
+      stream_store(&val_c[off + r], slot < (uint32_t)S ? vals[slot] : V(0));
     }
     __syncthreads();
   }
@@ -2030,8 +2030,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(
     }
     __syncthreads();
     for (int32_t i = tid; i < z; i += T) {
-      col_c[off + i] = (int32_t)key[i];
-      val_c[off + i] = vals[i];
+      stream_store(&col_c[off + i], (int32_t)key[i]);
+      stream_store(&val_c[off + i], vals[i]);
     }
     __syncthreads();  // before the next row reuses the LDS
     mv = nmv;
