@@ -26,11 +26,13 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
 }
 
 // Store of write-once output (C entries stored in coalesced runs by the
-// dense chunks, ESC and the LDS-hash numeric; the kept row bitmaps): with
+// dense chunks, ESC, the LDS-hash numeric and the light kernel's values;
+// the kept row bitmaps): with
 // AMS_NT_STORES set it is a non-temporal store, which streams
 // past the caches instead of evicting the B blocks and columns the kernels
-// re-read.  Only for runs of consecutive entries: the light kernel's
-// scattered per-bit column stores took 20 -> 57 ms on G500 s19 ef32 with it.
+// re-read.  Only for runs of consecutive entries: with it the light kernel's
+// per-bit column stores (one entry a lane, scattered) took 20 -> 57 ms on
+// G500 s19 ef32.
 #ifndef AMS_NT_STORES
 #define AMS_NT_STORES 1
 #endif

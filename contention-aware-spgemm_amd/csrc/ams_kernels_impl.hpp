@@ -3302,7 +3302,7 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
       int32_t* colp = col_c + d.off;
       V* valp = val_c + d.off;
       for (int i = lane; i < d.cnt; i += kWave) {
-        valp[i] = vals[i];
+        stream_store(&valp[i], vals[i]);
         vals[i] = V(0);
       }
       int32_t r = 0;
