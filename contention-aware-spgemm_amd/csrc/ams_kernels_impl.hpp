@@ -3312,17 +3312,22 @@ __global__ __launch_bounds__(kLightT) void k_numc_light(Csr A, int64_t mB, const
         for (int q = 0; q < kLightPbWpl; ++q) sum += __popc(ww0[q]);
         r = wave_incl_scan(sum) - sum;
       }
+      // each set bit's chunk column goes to its rank in the staging row (the
+      // segment table, read by now), then the lanes store consecutive
+      // entries: coalesced, where the bit loop's own stores were scattered
 #pragma unroll
       for (int q = 0; q < kLightPbWpl; ++q) {
         uint32_t x = ww0[q];
-        const int32_t cb = d.colbase + (lane * wpl + q) * 32;
+        const int32_t cl = (lane * wpl + q) * 32;
         while (x) {
-          if (r < d.cnt) colp[r] = cb + __builtin_ctz(x);
+          if (r < d.cnt) pre[r] = (uint16_t)(cl + __builtin_ctz(x));
           ++r;
           x &= x - 1u;
         }
       }
-      wave_lds_sync();  // vals and wp are read before the next morsel writes them
+      wave_lds_sync();
+      for (int i = lane; i < d.cnt; i += kWave) stream_store(&colp[i], d.colbase + (int32_t)pre[i]);
+      wave_lds_sync();  // vals, wp and the staging row are read before the next morsel writes them
       d0 = d1;
       d1 = d2;
       dv2 = dv3;
